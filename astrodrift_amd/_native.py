@@ -1,0 +1,425 @@
+"""ctypes binding of libdrift_hip.so (the C ABI in include/drift_hip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded
+every integration raises :class:`NativeLibraryError`; there is no fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import threading
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libdrift_hip.so"
+HEADER_PATH = Path(__file__).resolve().parent.parent / "include" / "drift_hip.h"
+
+# drift_status
+DRIFT_OK = 0
+DRIFT_ERR_INVALID_ARGUMENT = 1
+DRIFT_ERR_UNSUPPORTED = 2
+DRIFT_ERR_DEVICE = 3
+DRIFT_ERR_INTEGRATION = 4
+DRIFT_ERR_OUT_OF_MEMORY = 5
+
+METHOD_DOPR54 = 0
+METHOD_DOP853 = 1
+
+POTENTIAL_GALPY_KEPLER = 0
+POTENTIAL_KEPLER = 1
+POTENTIAL_PLUMMER = 2
+POTENTIAL_MW = 3
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+SCHEDULE_PLAIN = 0
+SCHEDULE_PERSISTENT = 1
+
+STATUS_CODE_MASK = 0xFF
+STATUS_NONFINITE = 0x100
+STATUS_STEP_LIMIT = 0x200
+
+# src/integrators/galpy/mod.rs:8
+INITIAL_STEP_SENTINEL = -9999.99
+
+
+class NativeLibraryError(RuntimeError):
+    """libdrift_hip.so is missing or failed to load."""
+
+
+class DriftNativeError(RuntimeError):
+    """A C-ABI call returned a non-zero drift_status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class CPotential(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("amp", ctypes.c_double),
+        ("b", ctypes.c_double),
+        ("bulge_amp", ctypes.c_double),
+        ("bulge_alpha", ctypes.c_double),
+        ("bulge_rc", ctypes.c_double),
+        ("disk_amp", ctypes.c_double),
+        ("disk_a", ctypes.c_double),
+        ("disk_b", ctypes.c_double),
+        ("halo_amp", ctypes.c_double),
+        ("halo_a", ctypes.c_double),
+    ]
+
+
+class COrbitStats(ctypes.Structure):
+    _fields_ = [
+        ("accepted", ctypes.c_uint32),
+        ("rejected", ctypes.c_uint32),
+        ("dense_steps", ctypes.c_uint32),
+        ("status", ctypes.c_int32),
+    ]
+
+
+STATS_DTYPE = np.dtype(
+    [("accepted", "<u4"), ("rejected", "<u4"), ("dense_steps", "<u4"), ("status", "<i4")]
+)
+
+
+class CLaunchOptions(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("schedule", ctypes.c_int32),
+        ("max_attempts", ctypes.c_uint32),
+    ]
+
+
+# ---------------------------------------------------------------------------
+# Potential descriptors
+# ---------------------------------------------------------------------------
+
+
+def _lower_gamma(a: float, x: float) -> float:
+    """gamma(a, x), series (A&S 6.5.29) / Lentz continued fraction (6.5.31)."""
+    if x <= 0.0:
+        return 0.0
+    if x < a + 1.0:
+        ap, term, total = a, 1.0 / a, 1.0 / a
+        for _ in range(1000):
+            ap += 1.0
+            term *= x / ap
+            total += term
+            if abs(term) < abs(total) * 1e-17:
+                break
+        return total * math.exp(-x + a * math.log(x))
+    tiny = 1e-300
+    b = x + 1.0 - a
+    c = 1.0 / tiny
+    d = 1.0 / b
+    h = d
+    for i in range(1, 1000):
+        an = -i * (i - a)
+        b += 2.0
+        d = an * d + b
+        d = tiny if abs(d) < tiny else d
+        c = b + an / c
+        c = tiny if abs(c) < tiny else c
+        d = 1.0 / d
+        delta = d * c
+        h *= delta
+        if abs(delta - 1.0) < 1e-17:
+            break
+    return math.gamma(a) - math.exp(-x + a * math.log(x)) * h
+
+
+@dataclass(frozen=True)
+class PotentialSpec:
+    """A force model the kernels evaluate (G = 1 natural units)."""
+
+    kind: int
+    amp: float = 0.0
+    b: float = 0.0
+    bulge_amp: float = 0.0
+    bulge_alpha: float = 0.0
+    bulge_rc: float = 0.0
+    disk_amp: float = 0.0
+    disk_a: float = 0.0
+    disk_b: float = 0.0
+    halo_amp: float = 0.0
+    halo_a: float = 0.0
+
+    def to_c(self) -> CPotential:
+        return CPotential(
+            self.kind,
+            0,
+            self.amp,
+            self.b,
+            self.bulge_amp,
+            self.bulge_alpha,
+            self.bulge_rc,
+            self.disk_amp,
+            self.disk_a,
+            self.disk_b,
+            self.halo_amp,
+            self.halo_a,
+        )
+
+    @staticmethod
+    def galpy_kepler() -> "PotentialSpec":
+        """The reference's hard-coded unit Kepler force (dopr54.rs:75-88)."""
+        return PotentialSpec(POTENTIAL_GALPY_KEPLER)
+
+    @staticmethod
+    def kepler(amp: float) -> "PotentialSpec":
+        return PotentialSpec(POTENTIAL_KEPLER, amp=float(amp))
+
+    @staticmethod
+    def plummer(amp: float, b: float) -> "PotentialSpec":
+        return PotentialSpec(POTENTIAL_PLUMMER, amp=float(amp), b=float(b))
+
+    @staticmethod
+    def milky_way(
+        bulge_alpha: float = 1.8,
+        bulge_rc: float = 1.9 / 8.0,
+        bulge_frac: float = 0.05,
+        disk_a: float = 3.0 / 8.0,
+        disk_b: float = 0.28 / 8.0,
+        disk_frac: float = 0.6,
+        halo_a: float = 16.0 / 8.0,
+        halo_frac: float = 0.35,
+    ) -> "PotentialSpec":
+        """Bulge + Miyamoto-Nagai disk + NFW halo normalised so that each
+        component contributes `*_frac` of vc^2 at R = 1, z = 0 (vc(1) = 1 when
+        the fractions sum to 1).  The defaults are the model behind the
+        reference's BovyPotential (shared/src/potential.rs:94-114)."""
+        disk_amp = disk_frac * (1.0 + (disk_a + disk_b) ** 2) ** 1.5
+        halo_amp = halo_frac / (math.log(1.0 + 1.0 / halo_a) - 1.0 / (halo_a + 1.0))
+        a = 1.5 - 0.5 * bulge_alpha
+        bulge_amp = bulge_frac / (
+            2.0 * math.pi * bulge_rc ** (3.0 - bulge_alpha) * _lower_gamma(a, 1.0 / bulge_rc**2)
+        )
+        return PotentialSpec(
+            POTENTIAL_MW,
+            bulge_amp=bulge_amp,
+            bulge_alpha=bulge_alpha,
+            bulge_rc=bulge_rc,
+            disk_amp=disk_amp,
+            disk_a=disk_a,
+            disk_b=disk_b,
+            halo_amp=halo_amp,
+            halo_a=halo_a,
+        )
+
+
+# ---------------------------------------------------------------------------
+# Library loading
+# ---------------------------------------------------------------------------
+
+_lib = None
+_lib_lock = threading.Lock()
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    c = ctypes
+    lib.drift_get_abi_version.restype = c.c_int
+    lib.drift_get_abi_version.argtypes = []
+    lib.drift_last_error.restype = c.c_char_p
+    lib.drift_last_error.argtypes = []
+    lib.drift_session_create.restype = c.c_int
+    lib.drift_session_create.argtypes = [c.c_int, c.POINTER(c.c_void_p)]
+    lib.drift_session_destroy.restype = c.c_int
+    lib.drift_session_destroy.argtypes = [c.c_void_p]
+    host = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_void_p, c.c_size_t, c.c_size_t,
+            c.c_double, c.c_double]
+    lib.drift_galpy_dopr54.restype = c.c_int
+    lib.drift_galpy_dopr54.argtypes = host + [c.c_double, c.POINTER(CPotential), c.c_void_p]
+    lib.drift_galpy_dop853.restype = c.c_int
+    lib.drift_galpy_dop853.argtypes = host + [c.POINTER(CPotential), c.c_void_p]
+    lib.drift_galpy_dopr54_device.restype = c.c_int
+    lib.drift_galpy_dopr54_device.argtypes = host + [
+        c.c_double, c.POINTER(CPotential), c.c_void_p, c.POINTER(CLaunchOptions), c.c_void_p]
+    lib.drift_galpy_dop853_device.restype = c.c_int
+    lib.drift_galpy_dop853_device.argtypes = host + [
+        c.POINTER(CPotential), c.c_void_p, c.POINTER(CLaunchOptions), c.c_void_p]
+    lib.drift_force_device.restype = c.c_int
+    lib.drift_force_device.argtypes = [c.c_void_p, c.POINTER(CPotential), c.c_void_p, c.c_void_p,
+                                       c.c_size_t, c.c_int32, c.c_void_p]
+    lib.drift_mw_table_check.restype = c.c_int
+    lib.drift_mw_table_check.argtypes = [c.POINTER(CPotential), c.c_int32, _D]
+
+
+def load_library() -> ctypes.CDLL:
+    """Loads libdrift_hip.so once; raises NativeLibraryError if unavailable."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise NativeLibraryError(
+                    f"{LIB_PATH} is not built; run `python -m astrodrift_amd.build` "
+                    "(hipcc, gfx950)"
+                )
+            try:
+                lib = ctypes.CDLL(str(LIB_PATH))
+            except OSError as exc:  # pragma: no cover - environment specific
+                raise NativeLibraryError(f"cannot load {LIB_PATH}: {exc}") from exc
+            _declare(lib)
+            if lib.drift_get_abi_version() != 1:
+                raise NativeLibraryError("libdrift_hip.so ABI version mismatch")
+            _lib = lib
+        return _lib
+
+
+def last_error() -> str:
+    msg = load_library().drift_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(code: int) -> None:
+    if code != DRIFT_OK:
+        raise DriftNativeError(code, last_error() or f"drift status {code}")
+
+
+class Session:
+    """Owns a libdrift_hip session (device, stream and cached device buffers)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        handle = ctypes.c_void_p()
+        check(self.lib.drift_session_create(int(device), ctypes.byref(handle)))
+        self.handle = handle
+        self.device = int(device)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.drift_session_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order varies
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_sessions: dict[int, Session] = {}
+_sessions_lock = threading.Lock()
+
+
+def session(device: int = 0) -> Session:
+    """The process-wide session for `device` (created on first use)."""
+    with _sessions_lock:
+        s = _sessions.get(device)
+        if s is None:
+            s = Session(device)
+            _sessions[device] = s
+        return s
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _as_state(state0) -> np.ndarray:
+    a = np.ascontiguousarray(state0, dtype=np.float64)
+    if a.ndim == 1:
+        if a.size % 6:
+            raise ValueError("state0 must have 6N values")
+        a = a.reshape(-1, 6)
+    if a.ndim != 2 or a.shape[1] != 6:
+        raise ValueError("state0 must have shape (N, 6)")
+    return a
+
+
+def launch_host(
+    method: int,
+    state0,
+    times,
+    rtol_ln: float,
+    atol_ln: float,
+    potential: PotentialSpec | None = None,
+    dt_one: float = INITIAL_STEP_SENTINEL,
+    device: int = 0,
+    raise_on_integration_error: bool = True,
+):
+    """Host-buffer launch (drift_galpy_dopr54 / drift_galpy_dop853).
+
+    Returns (out, stats, status): out has shape (nt, n, 6) time-major, stats a
+    structured array of drift_orbit_stats, status the drift_status code.
+    """
+    s = session(device)
+    y0 = _as_state(state0)
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    n, nt = y0.shape[0], t.shape[0]
+    out = np.empty((nt, n, 6), dtype=np.float64)
+    stats = np.zeros(n, dtype=STATS_DTYPE)
+    pot = potential.to_c() if potential is not None else None
+    pot_ref = ctypes.byref(pot) if pot is not None else None
+    if method == METHOD_DOPR54:
+        code = s.lib.drift_galpy_dopr54(s.handle, _ptr(y0), _ptr(t), _ptr(out), n, nt,
+                                        rtol_ln, atol_ln, dt_one, pot_ref, _ptr(stats))
+    else:
+        code = s.lib.drift_galpy_dop853(s.handle, _ptr(y0), _ptr(t), _ptr(out), n, nt,
+                                        rtol_ln, atol_ln, pot_ref, _ptr(stats))
+    if code != DRIFT_OK and (code != DRIFT_ERR_INTEGRATION or raise_on_integration_error):
+        check(code)
+    return out, stats, code
+
+
+def launch_device(
+    method: int,
+    d_state0: int,
+    d_times: int,
+    d_out: int,
+    n: int,
+    nt: int,
+    rtol_ln: float,
+    atol_ln: float,
+    potential: PotentialSpec | None = None,
+    d_stats: int = 0,
+    precision: int = PRECISION_FP64,
+    schedule: int = SCHEDULE_PLAIN,
+    block_size: int = 0,
+    max_attempts: int = 0,
+    stream: int = 0,
+    dt_one: float = INITIAL_STEP_SENTINEL,
+    device: int = 0,
+) -> None:
+    """Device-resident launch on raw device pointers (e.g. torch data_ptr())."""
+    s = session(device)
+    opts = CLaunchOptions(precision, block_size, schedule, max_attempts)
+    pot = potential.to_c() if potential is not None else None
+    pot_ref = ctypes.byref(pot) if pot is not None else None
+    args = (s.handle, ctypes.c_void_p(d_state0), ctypes.c_void_p(d_times), ctypes.c_void_p(d_out),
+            n, nt, rtol_ln, atol_ln)
+    if method == METHOD_DOPR54:
+        code = s.lib.drift_galpy_dopr54_device(*args, dt_one, pot_ref, ctypes.c_void_p(d_stats),
+                                               ctypes.byref(opts), ctypes.c_void_p(stream))
+    else:
+        code = s.lib.drift_galpy_dop853_device(*args, pot_ref, ctypes.c_void_p(d_stats),
+                                               ctypes.byref(opts), ctypes.c_void_p(stream))
+    check(code)
+
+
+def force_device(potential: PotentialSpec | None, d_pos: int, d_acc: int, n: int,
+                 precision: int = PRECISION_FP64, stream: int = 0, device: int = 0) -> None:
+    s = session(device)
+    pot = potential.to_c() if potential is not None else None
+    check(s.lib.drift_force_device(s.handle, ctypes.byref(pot) if pot is not None else None,
+                                   ctypes.c_void_p(d_pos), ctypes.c_void_p(d_acc), n, precision,
+                                   ctypes.c_void_p(stream)))
+
+
+def mw_table_check(potential: PotentialSpec, samples_per_segment: int = 200) -> float:
+    """Worst relative error of the MW spherical-force table (host only)."""
+    lib = load_library()
+    out = ctypes.c_double()
+    pot = potential.to_c()
+    check(lib.drift_mw_table_check(ctypes.byref(pot), samples_per_segment, ctypes.byref(out)))
+    return out.value

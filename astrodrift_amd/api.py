@@ -1,0 +1,547 @@
+"""The reference's Python API (``dft.Config`` / ``sim.add`` / ``sim.run``) on
+the gfx950 engine.
+
+Mirrors src/interface/{mod,container,recipe,error}.rs and
+python/drift/{__init__,selectors}.py: the same constructors, validation
+messages, result order and shape ``(nt, n, 6)``, and exception types.  The GPU
+engine runs every state-bearing container through libdrift_hip.so
+(drift_galpy_dopr54 / drift_galpy_dop853).  Differences from the reference,
+all deliberate:
+
+* the per-container particle limit (MAX_PARTICLES = 1000,
+  shared/src/lib.rs:25) is lifted, and the output-time limit is raised from
+  1024 to MAX_OUTPUT_TIMES;
+* ``Engine.CPU`` configurations validate as before but ``run()`` raises
+  NotImplementedError: this package replaces the GPU engine only;
+* ``Config(..., forces="model")`` integrates each container in the potential
+  of its force sources instead of the reference's hard-coded unit galpy Kepler
+  force (the default, ``forces="reference"``, keeps the reference behaviour,
+  src/dispatch/gpu.rs:155-156).
+"""
+
+from __future__ import annotations
+
+import itertools
+import logging
+import math
+import time
+import warnings
+from dataclasses import dataclass
+from typing import Any, Sequence
+
+import numpy as np
+
+from . import _native
+from .selectors import Engine, Implementation, Method
+
+MAX_MODEL_COMPONENTS = 11  # shared/src/lib.rs:19
+MAX_OUTPUT_TIMES = 1 << 20  # reference: 1024 (shared/src/lib.rs:27)
+MIN_RTOL = 1e-12  # shared/src/lib.rs:29
+MIN_ATOL = 1e-12  # shared/src/lib.rs:30
+INPUT_STATE_DIM = 6
+GRID_ULP_TOLERANCE = 8.0  # src/interface/mod.rs:68
+_EPS = np.finfo(np.float64).eps
+_MIN_POSITIVE = np.finfo(np.float64).tiny
+
+_log_gpu = logging.getLogger("drift.dispatch.gpu")
+_log_hip = logging.getLogger("drift.dispatch.gpu.hip")
+
+
+class DriftError(RuntimeError):
+    """Base class for operational errors reported by drift."""
+
+
+class IntegrationError(DriftError):
+    """An integration could not be completed."""
+
+
+DriftError.__module__ = "astrodrift_amd"
+IntegrationError.__module__ = "astrodrift_amd"
+
+
+# ---------------------------------------------------------------------------
+# Output grid and tolerance (src/interface/mod.rs:29-171, shared/src/config.rs)
+# ---------------------------------------------------------------------------
+
+
+@dataclass(frozen=True)
+class OutputGrid:
+    start: float = 0.0
+    end: float = 2.0 * math.pi
+    steps: int = 100
+
+    def sample(self, index: int) -> float:
+        """OutputGrid::sample, shared/src/config.rs:57-66."""
+        if index == self.steps - 1:
+            return self.end
+        return self.start + (self.end - self.start) * float(index) / float(self.steps - 1)
+
+    def times(self) -> np.ndarray:
+        """sample_times, src/dispatch/mod.rs:14-18."""
+        return np.array([self.sample(i) for i in range(self.steps)], dtype=np.float64)
+
+
+def _grid_new(start: float, end: float, steps: int) -> OutputGrid:
+    if not (2 <= steps <= MAX_OUTPUT_TIMES):
+        raise ValueError(f"ts must contain between 2 and {MAX_OUTPUT_TIMES} output times")
+    if not (math.isfinite(start) and math.isfinite(end)):
+        raise ValueError("ts endpoints must be finite")
+    span = end - start
+    if span == 0.0 or not math.isfinite(span):
+        raise ValueError("ts must span a non-zero finite interval")
+    return OutputGrid(float(start), float(end), int(steps))
+
+
+def _grid_from_times(times: Sequence[float]) -> OutputGrid:
+    times = [float(t) for t in times]
+    if not (2 <= len(times) <= MAX_OUTPUT_TIMES):
+        raise ValueError(f"ts must contain between 2 and {MAX_OUTPUT_TIMES} output times")
+    if any(not math.isfinite(t) for t in times):
+        raise ValueError("all ts values must be finite")
+    grid = _grid_new(times[0], times[-1], len(times))
+    increasing = grid.end > grid.start
+    for a, b in zip(times, times[1:]):
+        if (b <= a) if increasing else (b >= a):
+            raise ValueError("ts values must be strictly increasing or strictly decreasing")
+    step = (grid.end - grid.start) / float(grid.steps - 1)
+    for index, actual in enumerate(times):
+        expected = grid.sample(index)
+        scale = max(abs(actual), abs(expected), abs(step), _MIN_POSITIVE)
+        if abs(actual - expected) > GRID_ULP_TOLERANCE * _EPS * scale:
+            raise ValueError(
+                "ts values must be uniformly spaced; arbitrary output times are not yet supported"
+            )
+    return grid
+
+
+def _parse_grid(obj: Any) -> OutputGrid:
+    """PyOutputGrid::extract, src/interface/mod.rs:31-65."""
+    if isinstance(obj, tuple):
+        if len(obj) != 3:
+            raise ValueError("time-grid tuple must have 3 elements: (start, end, steps)")
+        start, end, steps = obj
+        if isinstance(steps, bool) or not isinstance(steps, (int, np.integer)):
+            raise TypeError("time-grid steps must be an integer")
+        if steps < 0:
+            raise OverflowError("time-grid steps must be non-negative")
+        return _grid_new(float(start), float(end), int(steps))
+    if isinstance(obj, np.ndarray) and obj.dtype == np.float64 and obj.ndim == 1:
+        if not obj.flags.c_contiguous:
+            raise ValueError("NumPy time arrays must be contiguous")
+        return _grid_from_times(obj.tolist())
+    try:
+        seq = [float(v) for v in obj]
+    except TypeError:
+        raise ValueError(
+            "Expected (start, end, steps) tuple or 1D numpy.linspace array"
+        ) from None
+    return _grid_from_times(seq)
+
+
+def _parse_tolerance(obj: Any) -> tuple[float, float]:
+    """PyTolerance::extract, src/interface/mod.rs:139-171."""
+    if isinstance(obj, tuple) and len(obj) == 2:
+        rtol, atol = float(obj[0]), float(obj[1])
+    else:
+        try:
+            rtol = atol = float(obj)
+        except (TypeError, ValueError):
+            raise ValueError(
+                "Expected (rtol, atol) tuple or single float tolerance value"
+            ) from None
+    if not (math.isfinite(rtol) and math.isfinite(atol)) or rtol <= 0.0 or atol <= 0.0:
+        raise ValueError("rtol and atol must be finite and greater than zero")
+    return rtol, atol
+
+
+def log_tolerance(rtol: float, atol: float) -> tuple[float, float]:
+    """LogTolerance::from_linear, src/integrators/galpy/mod.rs:19-24."""
+    return math.log(rtol), math.log(atol)
+
+
+def _selector(value: Any, cls: type, name: str):
+    """extract_value, src/interface/selector.rs:4-22."""
+    if value is None:
+        return list(cls)[0]
+    if type(value) is not cls:
+        raise TypeError(f"expected a drift.{name} member (astrodrift_amd.{name})")
+    return value
+
+
+def _rust_display(x: float) -> str:
+    if math.isfinite(x) and x == int(x) and abs(x) < 1e16:
+        return str(int(x))
+    return repr(float(x))
+
+
+# ---------------------------------------------------------------------------
+# Potentials and containers (src/interface/recipe.rs, container.rs)
+# ---------------------------------------------------------------------------
+
+
+def _python_float(value: float) -> str:
+    if math.isnan(value):
+        return "float('nan')"
+    if value == math.inf:
+        return "float('inf')"
+    if value == -math.inf:
+        return "float('-inf')"
+    return repr(float(value))
+
+
+class Potential:
+    """A gravitational potential definition.
+
+    Construct potentials with :meth:`Potential.kepler`,
+    :meth:`Potential.plummer`, or :meth:`Potential.bovy`.
+    """
+
+    __slots__ = ("_name", "_amp", "_radius")
+
+    def __init__(self, name: str = "kepler", amp: float = 1.0, radius: float = 0.0):
+        self._name = name
+        self._amp = float(amp)
+        self._radius = float(radius)
+
+    def __repr__(self) -> str:
+        if self._name == "kepler":
+            return f"Potential.kepler(amp={_python_float(self._amp)})"
+        if self._name == "plummer":
+            return (
+                f"Potential.plummer(amp={_python_float(self._amp)}, "
+                f"radius={_python_float(self._radius)})"
+            )
+        return "Potential.bovy()"
+
+    @staticmethod
+    def kepler(amp: float) -> "Potential":
+        """Point-mass potential (G = 1); ``amp`` is the total mass."""
+        return Potential("kepler", float(amp))
+
+    @staticmethod
+    def plummer(amp: float, radius: float) -> "Potential":
+        """Plummer sphere with total mass ``amp`` and scale radius ``radius``."""
+        return Potential("plummer", float(amp), float(radius))
+
+    @staticmethod
+    def bovy() -> "Potential":
+        """The built-in composite background potential: cut-off power-law
+        bulge + Miyamoto-Nagai disk + NFW halo (shared/src/potential.rs:87-124)."""
+        return Potential("bovy")
+
+    def spec(self) -> _native.PotentialSpec:
+        """The kernel force model for this potential."""
+        if self._name == "kepler":
+            return _native.PotentialSpec.kepler(self._amp)
+        if self._name == "plummer":
+            return _native.PotentialSpec.plummer(self._amp, self._radius)
+        return _native.PotentialSpec.milky_way()
+
+
+_next_identity = itertools.count()
+
+
+class Container:
+    """A state-bearing particle group or stationary background potential."""
+
+    __slots__ = ("_num_particles", "_potential", "_state", "_identity")
+
+    def __init__(self, num_particles, potential, state):
+        self._num_particles = num_particles
+        self._potential = potential
+        self._state = state
+        self._identity = next(_next_identity)
+
+    @property
+    def num_particles(self) -> int | None:
+        """Number of particles in this group (``None`` for background containers)."""
+        return self._num_particles
+
+    def __repr__(self) -> str:
+        if self._num_particles is not None and self._potential is not None:
+            return (
+                f"Container(kind='particles', num_particles={self._num_particles}, "
+                f"potential={self._potential!r})"
+            )
+        if self._num_particles is not None:
+            return f"Container(kind='test_particles', num_particles={self._num_particles})"
+        if self._potential is not None:
+            return f"Container(kind='background', potential={self._potential!r})"
+        return "Container(kind='background', potential=None)"
+
+
+def _validate_istate(istate: Any) -> np.ndarray:
+    """validate_istate, src/interface/container.rs:59-93 (no particle limit)."""
+    if not isinstance(istate, np.ndarray):
+        raise TypeError("istate must be a numpy.ndarray of float64")
+    if istate.dtype != np.float64:
+        raise TypeError("istate must be a float64 array")
+    shape = istate.shape
+    if istate.ndim == 1:
+        particles, ok = shape[0] // INPUT_STATE_DIM, shape[0] % INPUT_STATE_DIM == 0
+    elif istate.ndim == 2:
+        particles, ok = shape[0], shape[1] == INPUT_STATE_DIM
+    else:
+        particles, ok = 0, False
+    if not ok:
+        raise ValueError(
+            "istate must have shape (N, 6) with columns [x, y, z, vx, vy, vz] "
+            f"(or a flat array of length 6N); got shape {list(shape)}"
+        )
+    if particles == 0:
+        raise ValueError("istate must contain at least one particle")
+    if not np.all(np.isfinite(istate)):
+        raise ValueError("istate positions and velocities must be finite")
+    state = np.array(istate, dtype=np.float64, order="C").reshape(particles, INPUT_STATE_DIM)
+    state.flags.writeable = False
+    return state
+
+
+def test_particles(istate: np.ndarray) -> Container:
+    """A state-bearing particle group without an attached potential."""
+    state = _validate_istate(istate)
+    return Container(state.shape[0], None, state)
+
+
+test_particles.__test__ = False  # not a pytest test despite its name
+
+
+def particles(potential: Potential, istate: np.ndarray) -> Container:
+    """A state-bearing particle group with an attached potential."""
+    if not isinstance(potential, Potential):
+        raise TypeError("potential must be a Potential")
+    if potential._name not in ("kepler", "plummer"):
+        raise NotImplementedError(
+            "only Kepler and Plummer potentials can currently be attached to particles"
+        )
+    state = _validate_istate(istate)
+    return Container(state.shape[0], potential, state)
+
+
+def background(potential: Potential) -> Container:
+    """A stationary background potential for use as a force source."""
+    if not isinstance(potential, Potential):
+        raise TypeError("potential must be a Potential")
+    return Container(None, potential, None)
+
+
+# ---------------------------------------------------------------------------
+# Config (src/interface/mod.rs:178-469)
+# ---------------------------------------------------------------------------
+
+
+def validate_configuration(engine: Engine, method: Method, implementation: Implementation) -> None:
+    """validate_configuration, src/integrators/mod.rs:23-36."""
+    if implementation is not Implementation.GALPY:
+        raise NotImplementedError(
+            f"{engine.value} + {implementation.value} {method.value} is not implemented"
+        )
+
+
+class Config:
+    """Configuration and registered container graph for an integration.
+
+    The default selectors are ``Engine.CPU``, ``Method.DOPR54``, and
+    ``Implementation.GALPY``.  Register force relationships with :meth:`add`,
+    then execute the complete model with :meth:`run`.
+    """
+
+    def __init__(
+        self,
+        engine: Engine | None = None,
+        method: Method | None = None,
+        implementation: Implementation | None = None,
+        ts: Any = None,
+        tolerance: Any = None,
+        *,
+        forces: str = "reference",
+        device: int = 0,
+    ):
+        self.engine = _selector(engine, Engine, "Engine")
+        self.method = _selector(method, Method, "Method")
+        self.implementation = _selector(implementation, Implementation, "Implementation")
+        self.grid = OutputGrid() if ts is None else _parse_grid(ts)
+        self.rtol, self.atol = (MIN_RTOL, MIN_ATOL) if tolerance is None else _parse_tolerance(tolerance)
+        if forces not in ("reference", "model"):
+            raise ValueError("forces must be 'reference' or 'model'")
+        self.forces = forces
+        self.device = int(device)
+        validate_configuration(self.engine, self.method, self.implementation)
+        self._dependencies: list[tuple[int, int]] = []  # (source, dependent)
+        self._containers: list[Container] = []
+
+    # -- graph registration -------------------------------------------------
+
+    def _path_exists(self, start: int, target: int) -> bool:
+        pending, visited = [start], set()
+        while pending:
+            current = pending.pop()
+            if current == target:
+                return True
+            if current in visited:
+                continue
+            visited.add(current)
+            pending.extend(dep for src, dep in self._dependencies if src == current)
+        return False
+
+    def add(self, node: Container, *requires: Container) -> None:
+        """Register that ``node`` is integrated with ``requires`` as inputs."""
+        if not isinstance(node, Container) or not all(isinstance(r, Container) for r in requires):
+            raise TypeError("add() takes Container arguments")
+        unique: list[Container] = []
+        for r in requires:
+            if all(r._identity != u._identity for u in unique):
+                unique.append(r)
+        containers = list(self._containers)
+        for c in [node, *unique]:
+            if all(c._identity != k._identity for k in containers):
+                containers.append(c)
+        if len(containers) > MAX_MODEL_COMPONENTS:
+            raise ValueError(
+                f"add() would register {len(containers)} containers but a model supports "
+                f"at most {MAX_MODEL_COMPONENTS}"
+            )
+        if not unique:
+            raise ValueError("add() requires at least one dependency")
+        if any(self._path_exists(node._identity, r._identity) for r in unique):
+            raise ValueError("adding these dependencies would create a cycle")
+        for r in unique:
+            edge = (r._identity, node._identity)
+            if edge not in self._dependencies:
+                self._dependencies.append(edge)
+        self._containers = containers
+
+    def dependency(self, node: Container, *requires: Container) -> None:
+        """Deprecated alias for :meth:`add`."""
+        warnings.warn(
+            "Config.dependency() is deprecated; use Config.add(node, *requires)",
+            DeprecationWarning,
+            stacklevel=2,
+        )
+        self.add(node, *requires)
+
+    def info(self) -> str:
+        """Return a human-readable summary of this configuration."""
+        return (
+            f"Config(engine={self.engine.value}, method={self.method.value}, "
+            f"implementation={self.implementation.value}, "
+            f"times=({_rust_display(self.grid.start)}, {_rust_display(self.grid.end)}, "
+            f"{self.grid.steps}), tolerance=(rtol={self.rtol:.6e}, atol={self.atol:.6e}), "
+            f"containers={len(self._containers)}, dependencies={len(self._dependencies)})"
+        )
+
+    def __repr__(self) -> str:
+        return self.info()
+
+    # -- execution ----------------------------------------------------------
+
+    def _force_for(self, node: Container) -> _native.PotentialSpec | None:
+        if self.forces == "reference":
+            return None  # unit galpy Kepler, as the reference kernels hard-code
+        sources = [
+            c
+            for c in self._containers
+            if (c._identity, node._identity) in self._dependencies
+        ]
+        statics = []
+        for c in sources:
+            if c._state is not None and c._potential is not None:
+                raise NotImplementedError(
+                    "particle groups as force sources need moving-origin potentials, "
+                    "which are not supported"
+                )
+            if c._potential is not None:
+                statics.append(c._potential)
+        if len(statics) != 1:
+            raise NotImplementedError(
+                "forces='model' needs exactly one background potential per integrated container"
+            )
+        return statics[0].spec()
+
+    def run(self) -> list:
+        """Integrate the registered model.
+
+        Results follow first-registration order.  Each state-bearing container
+        produces a float64 array shaped ``(time, particle, 6)``; stationary
+        background containers produce ``None``.
+
+        Raises ``IntegrationError`` when the backend cannot complete the
+        integration.
+        """
+        for c in self._containers:
+            if c._state is not None and not any(dep == c._identity for _, dep in self._dependencies):
+                raise ValueError(
+                    "each particle container must be registered with add(node, *requires) "
+                    "before integration"
+                )
+        if not any(c._state is not None for c in self._containers):
+            return [None for _ in self._containers]
+        if self.engine is Engine.CPU:
+            raise NotImplementedError(
+                "astrodrift_amd provides the GPU engine only; use Engine.GPU"
+            )
+        times = self.grid.times()
+        rtol_ln, atol_ln = log_tolerance(self.rtol, self.atol)
+        method = _native.METHOD_DOPR54 if self.method is Method.DOPR54 else _native.METHOD_DOP853
+        _log_gpu.info(
+            "GPU integration starting: implementation=%s, method=%s, %d output times over "
+            "[%s, %s], rtol=%.2e, atol=%.2e",
+            self.implementation.value,
+            self.method.value,
+            self.grid.steps,
+            _rust_display(self.grid.start),
+            _rust_display(self.grid.end),
+            self.rtol,
+            self.atol,
+        )
+        started = time.perf_counter()
+        results: list = []
+        stage = 0
+        total = 0
+        for c in self._containers:
+            if c._state is None:
+                results.append(None)
+                continue
+            stage += 1
+            n = c._state.shape[0]
+            total += n
+            _log_gpu.debug("stage %d: dispatching %d particles to the GPU", stage, n)
+            stage_started = time.perf_counter()
+            potential = self._force_for(c)
+            _log_hip.debug(
+                "launching GALPY %s kernel: %d particles, %d output times", self.method.value, n,
+                self.grid.steps,
+            )
+            try:
+                out, _stats, _code = _native.launch_host(
+                    method, c._state, times, rtol_ln, atol_ln, potential, device=self.device
+                )
+            except _native.DriftNativeError as exc:
+                raise IntegrationError(str(exc)) from exc
+            results.append(out)
+            _log_gpu.debug(
+                "stage %d: %d particles finished in %.3fs", stage, n,
+                time.perf_counter() - stage_started,
+            )
+        _log_gpu.info(
+            "GPU integration finished: %d stage(s), %d particle trajectories in %.3fs",
+            stage,
+            total,
+            time.perf_counter() - started,
+        )
+        return results
+
+
+__all__ = [
+    "Config",
+    "Container",
+    "DriftError",
+    "IntegrationError",
+    "OutputGrid",
+    "Potential",
+    "background",
+    "log_tolerance",
+    "particles",
+    "test_particles",
+    "validate_configuration",
+]

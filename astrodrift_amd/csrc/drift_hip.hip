@@ -1,0 +1,596 @@
+// drift_hip.hip -- gfx950 kernels and the C ABI of libdrift_hip.so.
+//
+// Replaces the reference's CUDA device crate (kernels/src/lib.rs) and its
+// cuda-oxide / rust-cuda backends (src/dispatch/gpu/*.rs).  Layout in HBM:
+//   state0  (n, 6)      AoS rows, read once per orbit (3 x 16-byte loads)
+//   times   (nt,)       shared by every orbit (L2/L1 resident)
+//   out     (nt, n, 6)  time-major AoS; lane i writes its 48-byte record of
+//                       sample s at ((s * n) + i) * 6 as 3 x 16-byte stores
+//   stats   (n,)        drift_orbit_stats, written once per orbit
+//   table   MW spherical-force coefficients, staged per block in LDS
+// Per-orbit phase-space state, stage derivatives and dense-output
+// coefficients live in registers for the whole integration.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "drift_internal.hpp"
+#include "forces.hpp"
+#include "integrators.hpp"
+
+namespace drift {
+
+// ---------------------------------------------------------------------------
+// Device side
+// ---------------------------------------------------------------------------
+
+struct OutSink {
+    double* out;
+    int64_t n;
+    int64_t idx;
+    __device__ __forceinline__ void emit(int32_t s, const double* y) const {
+        double2* p = reinterpret_cast<double2*>(out + ((int64_t)s * n + idx) * DIM);
+        p[0] = make_double2(y[0], y[1]);
+        p[1] = make_double2(y[2], y[3]);
+        p[2] = make_double2(y[4], y[5]);
+    }
+};
+
+struct PlainArgs {
+    const double* state0;
+    const double* times;
+    double* out;
+    drift_orbit_stats* stats;
+    int64_t n;
+    int32_t nt;
+    uint32_t max_attempts;
+    double rtol, atol;    // linear tolerances
+    double dt_one_init;   // DOPR54 only
+    double bound2[6];     // DOPR54 only
+    PotParams pot;
+    const void* table;    // MW: TABLE_DOUBLES coefficients (double or float)
+};
+
+template <int KIND, class FT>
+struct ForceFor;
+
+template <class FT>
+struct ForceFor<POT_GALPY_KEPLER, FT> {
+    using type = GalpyKeplerForce<FT>;
+    __device__ static type make(const PotParams&, const FT*) { return type{}; }
+};
+template <class FT>
+struct ForceFor<POT_KEPLER, FT> {
+    using type = KeplerForce<FT>;
+    __device__ static type make(const PotParams& p, const FT*) { return type{FT(p.amp)}; }
+};
+template <class FT>
+struct ForceFor<POT_PLUMMER, FT> {
+    using type = PlummerForce<FT>;
+    __device__ static type make(const PotParams& p, const FT*) { return type{FT(p.amp), FT(p.b2)}; }
+};
+template <class FT>
+struct ForceFor<POT_MW, FT> {
+    using type = MwForce<FT>;
+    __device__ static type make(const PotParams& p, const FT* table) {
+        return type{&p, table, FT(p.disk_amp), FT(p.disk_a), FT(p.disk_b2)};
+    }
+};
+
+// Stages the MW coefficient table into LDS (no-op for the other potentials).
+template <int KIND, class FT>
+__device__ __forceinline__ const FT* stage_table(const void* gtable, FT* lds) {
+    if constexpr (KIND == POT_MW) {
+        const FT* g = static_cast<const FT*>(gtable);
+        for (int k = threadIdx.x; k < TABLE_DOUBLES; k += blockDim.x) lds[k] = g[k];
+        __syncthreads();
+        return lds;
+    } else {
+        return nullptr;
+    }
+}
+
+__device__ __forceinline__ void load_state(const double* state0, int64_t i, double* y) {
+    const double2* p = reinterpret_cast<const double2*>(state0 + i * DIM);
+    double2 a = p[0], b = p[1], c = p[2];
+    y[0] = a.x; y[1] = a.y; y[2] = b.x; y[3] = b.y; y[4] = c.x; y[5] = c.y;
+}
+
+constexpr int kDefaultBlock = 256;
+
+template <int KIND, class FT>
+__global__ void __launch_bounds__(kDefaultBlock) dop853_plain_kernel(PlainArgs a) {
+    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    const FT* table = stage_table<KIND, FT>(a.table, lds);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    const double t0 = a.times[0];
+    const Dop853Job J = make_dop853_job(a.times, a.nt, a.rtol, a.atol, a.max_attempts, t0,
+                                        a.times[a.nt - 1]);
+    OutSink sink{a.out, a.n, i};
+    double y0[DIM];
+    load_state(a.state0, i, y0);
+    Dop853Lane L;
+    dop853_init(L, y0, t0, J, f, sink);
+    while (!dop853_attempt(L, J, f, sink)) {
+    }
+    if (a.stats) {
+        drift_orbit_stats st;
+        st.accepted = L.accepted;
+        st.rejected = L.rejected;
+        st.dense_steps = L.dense;
+        st.status = L.status;
+        a.stats[i] = st;
+    }
+}
+
+template <int KIND, class FT>
+__global__ void __launch_bounds__(kDefaultBlock) dopr54_plain_kernel(PlainArgs a) {
+    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    const FT* table = stage_table<KIND, FT>(a.table, lds);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    Dopr54Job J;
+    J.times = a.times;
+    J.nt = a.nt;
+    J.max_attempts = a.max_attempts;
+    J.dt = a.times[1] - a.times[0];  // dopr54.rs:713
+    J.rtol = a.rtol;
+    J.atol = a.atol;
+    J.dt_one_init = a.dt_one_init;
+#pragma unroll
+    for (int k = 0; k < 6; k++) J.bound2[k] = a.bound2[k];
+    OutSink sink{a.out, a.n, i};
+    double y0[DIM];
+    load_state(a.state0, i, y0);
+    Dopr54Lane L;
+    dopr54_init(L, y0, a.times[0], J, f, sink);
+    bool done = L.ii >= J.nt - 1;
+    while (!done) done = dopr54_attempt(L, J, f, sink);
+    if (a.stats) {
+        drift_orbit_stats st;
+        st.accepted = L.accepted;
+        st.rejected = L.rejected;
+        st.dense_steps = 0;
+        st.status = L.err | L.status;
+        a.stats[i] = st;
+    }
+}
+
+struct ForceArgs {
+    const double* pos;
+    double* acc;
+    int64_t n;
+    PotParams pot;
+    const void* table;
+};
+
+template <int KIND, class FT>
+__global__ void __launch_bounds__(kDefaultBlock) force_kernel(ForceArgs a) {
+    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    const FT* table = stage_table<KIND, FT>(a.table, lds);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    double q[DIM] = {a.pos[3 * i], a.pos[3 * i + 1], a.pos[3 * i + 2], 0.0, 0.0, 0.0};
+    double d[DIM];
+    f(q, d);
+    a.acc[3 * i] = d[3];
+    a.acc[3 * i + 1] = d[4];
+    a.acc[3 * i + 2] = d[5];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct TableEntry {
+    drift_potential key;
+    PotParams params;
+    double* d_table64 = nullptr;
+    float* d_table32 = nullptr;
+};
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace drift
+
+struct drift_session {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<drift::TableEntry> tables;
+    drift::DeviceBuffer state0, times, out, stats;
+};
+
+namespace drift {
+namespace {
+
+#define DRIFT_HIP_TRY(expr)                                                              \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            set_last_error(std::string("HIP error: ") + hipGetErrorString(_e) + " in " + \
+                           #expr);                                                       \
+            return _e == hipErrorOutOfMemory ? DRIFT_ERR_OUT_OF_MEMORY : DRIFT_ERR_DEVICE; \
+        }                                                                                 \
+    } while (0)
+
+int fail(int code, const std::string& msg) {
+    set_last_error(msg);
+    return code;
+}
+
+bool finite(double x) { return std::isfinite(x); }
+
+int ensure_buffer(DeviceBuffer& b, size_t bytes) {
+    if (b.bytes >= bytes) return DRIFT_OK;
+    if (b.ptr) DRIFT_HIP_TRY(hipFree(b.ptr));
+    b.ptr = nullptr;
+    b.bytes = 0;
+    DRIFT_HIP_TRY(hipMalloc(&b.ptr, bytes));
+    b.bytes = bytes;
+    return DRIFT_OK;
+}
+
+bool same_potential(const drift_potential& a, const drift_potential& b) {
+    return std::memcmp(&a, &b, sizeof(drift_potential)) == 0;
+}
+
+int validate_potential(const drift_potential& p) {
+    if (p.reserved != 0) return fail(DRIFT_ERR_INVALID_ARGUMENT, "drift_potential.reserved must be 0");
+    switch (p.kind) {
+        case POT_GALPY_KEPLER:
+            return DRIFT_OK;
+        case POT_KEPLER:
+            if (!finite(p.amp)) return fail(DRIFT_ERR_INVALID_ARGUMENT, "Kepler amp must be finite");
+            return DRIFT_OK;
+        case POT_PLUMMER:
+            if (!finite(p.amp) || !finite(p.b) || p.b < 0.0)
+                return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                            "Plummer amp must be finite and radius finite and >= 0");
+            return DRIFT_OK;
+        case POT_MW: {
+            const double v[] = {p.bulge_amp, p.bulge_alpha, p.bulge_rc, p.disk_amp, p.disk_a,
+                                p.disk_b, p.halo_amp, p.halo_a};
+            for (double x : v)
+                if (!finite(x)) return fail(DRIFT_ERR_INVALID_ARGUMENT, "MW parameters must be finite");
+            if (!(p.bulge_alpha >= 0.0 && p.bulge_alpha < 3.0))
+                return fail(DRIFT_ERR_UNSUPPORTED, "MW bulge alpha must lie in [0, 3)");
+            if (!(p.bulge_rc > 0.0) || !(p.halo_a > 0.0) || p.disk_a < 0.0 || p.disk_b < 0.0)
+                return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                            "MW scale lengths must be positive (disk a, b may be 0)");
+            return DRIFT_OK;
+        }
+        default:
+            return fail(DRIFT_ERR_UNSUPPORTED, "unknown potential kind " + std::to_string(p.kind));
+    }
+}
+
+// Resolves a potential descriptor into kernel parameters (and, for MW, a
+// cached device table).  potential == NULL selects the reference's unit
+// galpy Kepler force.
+int resolve_potential(drift_session* s, const drift_potential* potential, PotParams& params,
+                      const void** table64, const void** table32) {
+    drift_potential p{};
+    p.kind = POT_GALPY_KEPLER;
+    if (potential) p = *potential;
+    int rc = validate_potential(p);
+    if (rc != DRIFT_OK) return rc;
+    *table64 = nullptr;
+    *table32 = nullptr;
+    params = PotParams{};
+    params.kind = p.kind;
+    if (p.kind == POT_KEPLER) {
+        params.amp = p.amp;
+    } else if (p.kind == POT_PLUMMER) {
+        params.amp = p.amp;
+        params.b2 = p.b * p.b;
+    } else if (p.kind == POT_MW) {
+        for (auto& e : s->tables) {
+            if (same_potential(e.key, p)) {
+                params = e.params;
+                *table64 = e.d_table64;
+                *table32 = e.d_table32;
+                return DRIFT_OK;
+            }
+        }
+        TableEntry e;
+        e.key = p;
+        build_mw_params(p, e.params);
+        std::vector<double> h64(TABLE_DOUBLES);
+        build_mw_table(p, h64.data());
+        std::vector<float> h32(TABLE_DOUBLES);
+        for (int k = 0; k < TABLE_DOUBLES; ++k) h32[k] = (float)h64[k];
+        DRIFT_HIP_TRY(hipMalloc(&e.d_table64, TABLE_DOUBLES * sizeof(double)));
+        DRIFT_HIP_TRY(hipMalloc(&e.d_table32, TABLE_DOUBLES * sizeof(float)));
+        DRIFT_HIP_TRY(hipMemcpy(e.d_table64, h64.data(), TABLE_DOUBLES * sizeof(double),
+                                hipMemcpyHostToDevice));
+        DRIFT_HIP_TRY(hipMemcpy(e.d_table32, h32.data(), TABLE_DOUBLES * sizeof(float),
+                                hipMemcpyHostToDevice));
+        s->tables.push_back(e);
+        params = e.params;
+        *table64 = e.d_table64;
+        *table32 = e.d_table32;
+    }
+    return DRIFT_OK;
+}
+
+using PlainKernel = void (*)(PlainArgs);
+
+template <int KIND>
+PlainKernel pick_plain(int method, int precision) {
+    if (method == DRIFT_METHOD_DOP853)
+        return precision == DRIFT_PRECISION_MIXED ? dop853_plain_kernel<KIND, float>
+                                                  : dop853_plain_kernel<KIND, double>;
+    return precision == DRIFT_PRECISION_MIXED ? dopr54_plain_kernel<KIND, float>
+                                              : dopr54_plain_kernel<KIND, double>;
+}
+
+PlainKernel plain_kernel_for(int method, int kind, int precision) {
+    switch (kind) {
+        case POT_GALPY_KEPLER: return pick_plain<POT_GALPY_KEPLER>(method, precision);
+        case POT_KEPLER: return pick_plain<POT_KEPLER>(method, precision);
+        case POT_PLUMMER: return pick_plain<POT_PLUMMER>(method, precision);
+        default: return pick_plain<POT_MW>(method, precision);
+    }
+}
+
+constexpr uint32_t kDefaultMaxAttempts = 1u << 22;
+constexpr double kMixedMinRtol = 1e-7;
+
+int launch_integrator(drift_session* s, int method, const double* d_state0, const double* d_times,
+                      double* d_out, size_t n, size_t nt, double rtol_ln, double atol_ln,
+                      double dt_one_init, const drift_potential* potential,
+                      drift_orbit_stats* d_stats, const drift_launch_options* options,
+                      hipStream_t stream) {
+    if (!s) return fail(DRIFT_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (n < 1) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n must be >= 1");
+    if (nt < 2) return fail(DRIFT_ERR_INVALID_ARGUMENT, "nt must be >= 2");
+    if (nt > (size_t)INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "nt too large");
+    if (!d_state0 || !d_times || !d_out)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "state0, times and state_out must be non-NULL");
+    if ((reinterpret_cast<uintptr_t>(d_state0) | reinterpret_cast<uintptr_t>(d_out)) & 15u)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "state0 and state_out must be 16-byte aligned");
+    if (!finite(rtol_ln) || !finite(atol_ln))
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "rtol and atol (natural logs) must be finite");
+    drift_launch_options opt{};
+    if (options) opt = *options;
+    if (opt.precision != DRIFT_PRECISION_FP64 && opt.precision != DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "unknown precision mode");
+    int block = opt.block_size ? opt.block_size : kDefaultBlock;
+    if (block < 64 || block > kDefaultBlock || block % 64 != 0)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
+    if (opt.schedule != 0)
+        return fail(DRIFT_ERR_UNSUPPORTED, "schedule must be 0 (plain grid)");
+
+    PlainArgs a{};
+    a.state0 = d_state0;
+    a.times = d_times;
+    a.out = d_out;
+    a.stats = d_stats;
+    a.n = (int64_t)n;
+    a.nt = (int32_t)nt;
+    a.max_attempts = opt.max_attempts ? opt.max_attempts : kDefaultMaxAttempts;
+    // glibc exp, as dop853.rs:340-341 applies to the logarithmic tolerances.
+    a.rtol = std::exp(rtol_ln);
+    a.atol = std::exp(atol_ln);
+    a.dt_one_init = dt_one_init;
+    dopr54_bounds(a.bound2);
+    if (opt.precision == DRIFT_PRECISION_MIXED && a.rtol < kMixedMinRtol * (1.0 - 1e-9))
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "mixed precision (FP32 stage forces) requires rtol >= 1e-7");
+
+    DRIFT_HIP_TRY(hipSetDevice(s->device));
+    const void *t64 = nullptr, *t32 = nullptr;
+    int rc = resolve_potential(s, potential, a.pot, &t64, &t32);
+    if (rc != DRIFT_OK) return rc;
+    a.table = opt.precision == DRIFT_PRECISION_MIXED ? t32 : t64;
+
+    PlainKernel k = plain_kernel_for(method, a.pot.kind, opt.precision);
+    const int64_t blocks = ((int64_t)n + block - 1) / block;
+    if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, stream ? stream : s->stream, a);
+    DRIFT_HIP_TRY(hipGetLastError());
+    return DRIFT_OK;
+}
+
+int host_launch(drift_session* s, int method, const double* state0, const double* times,
+                double* state_out, size_t n, size_t nt, double rtol_ln, double atol_ln,
+                double dt_one_init, const drift_potential* potential, drift_orbit_stats* stats) {
+    if (!s) return fail(DRIFT_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (!state0 || !times || !state_out)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "state0, times and state_out must be non-NULL");
+    if (n < 1) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n must be >= 1");
+    if (nt < 2) return fail(DRIFT_ERR_INVALID_ARGUMENT, "nt must be >= 2");
+    DRIFT_HIP_TRY(hipSetDevice(s->device));
+    const size_t in_bytes = n * DIM * sizeof(double);
+    const size_t out_bytes = nt * n * DIM * sizeof(double);
+    int rc;
+    if ((rc = ensure_buffer(s->state0, in_bytes)) != DRIFT_OK) return rc;
+    if ((rc = ensure_buffer(s->times, nt * sizeof(double))) != DRIFT_OK) return rc;
+    if ((rc = ensure_buffer(s->out, out_bytes)) != DRIFT_OK) return rc;
+    if ((rc = ensure_buffer(s->stats, n * sizeof(drift_orbit_stats))) != DRIFT_OK) return rc;
+    DRIFT_HIP_TRY(hipMemcpyAsync(s->state0.ptr, state0, in_bytes, hipMemcpyHostToDevice, s->stream));
+    DRIFT_HIP_TRY(hipMemcpyAsync(s->times.ptr, times, nt * sizeof(double), hipMemcpyHostToDevice,
+                                 s->stream));
+    rc = launch_integrator(s, method, (const double*)s->state0.ptr, (const double*)s->times.ptr,
+                           (double*)s->out.ptr, n, nt, rtol_ln, atol_ln, dt_one_init, potential,
+                           (drift_orbit_stats*)s->stats.ptr, nullptr, s->stream);
+    if (rc != DRIFT_OK) return rc;
+    std::vector<drift_orbit_stats> host_stats;
+    drift_orbit_stats* st = stats;
+    if (!st) {
+        host_stats.resize(n);
+        st = host_stats.data();
+    }
+    DRIFT_HIP_TRY(hipMemcpyAsync(state_out, s->out.ptr, out_bytes, hipMemcpyDeviceToHost, s->stream));
+    DRIFT_HIP_TRY(hipMemcpyAsync(st, s->stats.ptr, n * sizeof(drift_orbit_stats),
+                                 hipMemcpyDeviceToHost, s->stream));
+    DRIFT_HIP_TRY(hipStreamSynchronize(s->stream));
+    if (method == DRIFT_METHOD_DOPR54) {
+        // integrate_kepler maps a non-zero error code to Err(code)
+        // (dopr54.rs:123), reported as DispatchError::CpuIntegration.
+        for (size_t i = 0; i < n; ++i) {
+            const int code = st[i].status & DRIFT_STATUS_CODE_MASK;
+            if (code != 0)
+                return fail(DRIFT_ERR_INTEGRATION,
+                            "GPU DOPR54 integration failed with error code " + std::to_string(code));
+        }
+    }
+    return DRIFT_OK;
+}
+
+}  // namespace
+}  // namespace drift
+
+using namespace drift;
+
+extern "C" {
+
+int drift_get_abi_version(void) { return DRIFT_ABI_VERSION; }
+
+const char* drift_last_error(void) { return g_last_error.c_str(); }
+
+int drift_session_create(int device, drift_session** out) {
+    if (!out) return fail(DRIFT_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    DRIFT_HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                    "device " + std::to_string(device) + " out of range (" + std::to_string(count) +
+                        " devices)");
+    DRIFT_HIP_TRY(hipSetDevice(device));
+    drift_session* s = new drift_session();
+    s->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(DRIFT_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e));
+    }
+    *out = s;
+    return DRIFT_OK;
+}
+
+int drift_session_destroy(drift_session* s) {
+    if (!s) return DRIFT_OK;
+    (void)hipSetDevice(s->device);
+    (void)hipStreamSynchronize(s->stream);
+    for (auto& t : s->tables) {
+        (void)hipFree(t.d_table64);
+        (void)hipFree(t.d_table32);
+    }
+    for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats})
+        if (b->ptr) (void)hipFree(b->ptr);
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+    return DRIFT_OK;
+}
+
+int drift_galpy_dopr54(drift_session* session, const double* state0, const double* times,
+                       double* state_out, size_t n, size_t nt, double rtol, double atol,
+                       double dt_one_init, const drift_potential* potential,
+                       drift_orbit_stats* stats) {
+    return host_launch(session, DRIFT_METHOD_DOPR54, state0, times, state_out, n, nt, rtol, atol,
+                       dt_one_init, potential, stats);
+}
+
+int drift_galpy_dop853(drift_session* session, const double* state0, const double* times,
+                       double* state_out, size_t n, size_t nt, double rtol, double atol,
+                       const drift_potential* potential, drift_orbit_stats* stats) {
+    return host_launch(session, DRIFT_METHOD_DOP853, state0, times, state_out, n, nt, rtol, atol,
+                       dp54::initial_step_sentinel, potential, stats);
+}
+
+int drift_galpy_dopr54_device(drift_session* session, const double* d_state0,
+                              const double* d_times, double* d_state_out, size_t n, size_t nt,
+                              double rtol, double atol, double dt_one_init,
+                              const drift_potential* potential, drift_orbit_stats* d_stats,
+                              const drift_launch_options* options, void* stream) {
+    return launch_integrator(session, DRIFT_METHOD_DOPR54, d_state0, d_times, d_state_out, n, nt,
+                             rtol, atol, dt_one_init, potential, d_stats, options,
+                             (hipStream_t)stream);
+}
+
+int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
+                              const double* d_times, double* d_state_out, size_t n, size_t nt,
+                              double rtol, double atol, const drift_potential* potential,
+                              drift_orbit_stats* d_stats, const drift_launch_options* options,
+                              void* stream) {
+    return launch_integrator(session, DRIFT_METHOD_DOP853, d_state0, d_times, d_state_out, n, nt,
+                             rtol, atol, dp54::initial_step_sentinel, potential, d_stats, options,
+                             (hipStream_t)stream);
+}
+
+int drift_force_device(drift_session* session, const drift_potential* potential,
+                       const double* d_pos, double* d_acc, size_t n, int32_t precision,
+                       void* stream) {
+    if (!session) return fail(DRIFT_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (!d_pos || !d_acc || n < 1) return fail(DRIFT_ERR_INVALID_ARGUMENT, "bad force arguments");
+    if (precision != DRIFT_PRECISION_FP64 && precision != DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "unknown precision mode");
+    DRIFT_HIP_TRY(hipSetDevice(session->device));
+    ForceArgs a{};
+    a.pos = d_pos;
+    a.acc = d_acc;
+    a.n = (int64_t)n;
+    const void *t64 = nullptr, *t32 = nullptr;
+    int rc = resolve_potential(session, potential, a.pot, &t64, &t32);
+    if (rc != DRIFT_OK) return rc;
+    const bool mixed = precision == DRIFT_PRECISION_MIXED;
+    a.table = mixed ? t32 : t64;
+    using FK = void (*)(ForceArgs);
+    FK k;
+    switch (a.pot.kind) {
+        case POT_GALPY_KEPLER:
+            k = mixed ? force_kernel<POT_GALPY_KEPLER, float> : force_kernel<POT_GALPY_KEPLER, double>;
+            break;
+        case POT_KEPLER:
+            k = mixed ? force_kernel<POT_KEPLER, float> : force_kernel<POT_KEPLER, double>;
+            break;
+        case POT_PLUMMER:
+            k = mixed ? force_kernel<POT_PLUMMER, float> : force_kernel<POT_PLUMMER, double>;
+            break;
+        default:
+            k = mixed ? force_kernel<POT_MW, float> : force_kernel<POT_MW, double>;
+            break;
+    }
+    const int64_t blocks = ((int64_t)n + kDefaultBlock - 1) / kDefaultBlock;
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kDefaultBlock), 0,
+                       stream ? (hipStream_t)stream : session->stream, a);
+    DRIFT_HIP_TRY(hipGetLastError());
+    return DRIFT_OK;
+}
+
+int drift_mw_table_check(const drift_potential* potential, int32_t samples_per_segment,
+                         double* max_rel_error) {
+    if (!potential || !max_rel_error || samples_per_segment < 1)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "bad table-check arguments");
+    if (potential->kind != POT_MW) return fail(DRIFT_ERR_INVALID_ARGUMENT, "potential is not MW");
+    int rc = validate_potential(*potential);
+    if (rc != DRIFT_OK) return rc;
+    std::vector<double> coeffs(TABLE_DOUBLES);
+    build_mw_table(*potential, coeffs.data());
+    *max_rel_error = mw_table_max_rel_error(*potential, coeffs.data(), samples_per_segment);
+    return DRIFT_OK;
+}
+
+}  // extern "C"

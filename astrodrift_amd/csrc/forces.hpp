@@ -1,0 +1,89 @@
+// forces.hpp -- right-hand sides a = (v, F(x)) for the gfx950 integrators.
+//
+// Each force is a small value type whose operator() fills a[6] from q[6]; the
+// integrator cores are templated on it so the force is inlined into the step
+// with no indirect calls.  FT is the arithmetic type of the force itself:
+// double for the FP64 path, float for the mixed-precision path (positions are
+// rounded to float, accelerations widened back to double).
+#pragma once
+
+#include "drift_device.hpp"
+
+namespace drift {
+
+template <class FT>
+struct GalpyKeplerForce {
+    DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
+        FT ax, ay, az;
+        galpy_kepler_acc<FT>(FT(q[0]), FT(q[1]), FT(q[2]), ax, ay, az);
+        a[0] = q[3];
+        a[1] = q[4];
+        a[2] = q[5];
+        a[3] = double(ax);
+        a[4] = double(ay);
+        a[5] = double(az);
+    }
+};
+
+template <class FT>
+struct KeplerForce {
+    FT amp;
+    DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
+        FT ax, ay, az;
+        kepler_acc<FT>(amp, FT(q[0]), FT(q[1]), FT(q[2]), ax, ay, az);
+        a[0] = q[3];
+        a[1] = q[4];
+        a[2] = q[5];
+        a[3] = double(ax);
+        a[4] = double(ay);
+        a[5] = double(az);
+    }
+};
+
+template <class FT>
+struct PlummerForce {
+    FT amp, b2;
+    DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
+        FT ax, ay, az;
+        plummer_acc<FT>(amp, b2, FT(q[0]), FT(q[1]), FT(q[2]), ax, ay, az);
+        a[0] = q[3];
+        a[1] = q[4];
+        a[2] = q[5];
+        a[3] = double(ax);
+        a[4] = double(ay);
+        a[5] = double(az);
+    }
+};
+
+// Milky-Way model: Miyamoto-Nagai disk in closed form plus the tabulated
+// spherical part (bulge + NFW halo), see spherical_table.cpp.  `table` points
+// at TABLE_DOUBLES coefficients (in LDS inside the kernels).
+template <class FT>
+struct MwForce {
+    const PotParams* p;  // slow-path parameters (global/constant memory)
+    const FT* table;
+    FT disk_amp, disk_a, disk_b2;
+    DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
+        FT x = FT(q[0]), y = FT(q[1]), z = FT(q[2]);
+        FT dx, dy, dz;
+        mn_acc<FT>(disk_amp, disk_a, disk_b2, x, y, z, dx, dy, dz);
+        // spherical part, table lookup in double coordinates of r2
+        double r2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+        int seg;
+        double s;
+        FT S;
+        if (table_locate(r2, seg, s)) {
+            S = horner_table<FT>(table + seg * TABLE_NCOEF, FT(s));
+        } else {
+            S = FT(spherical_direct(*p, r2));
+        }
+        a[0] = q[3];
+        a[1] = q[4];
+        a[2] = q[5];
+        a[3] = double(dx - S * x);
+        a[4] = double(dy - S * y);
+        a[5] = double(dz - S * z);
+    }
+};
+
+}  // namespace drift

@@ -1,0 +1,669 @@
+// integrators.hpp -- per-lane DOPR54 and DOP853 step machines.
+//
+// One orbit per GPU lane.  Each integrator is split into init() and attempt():
+// attempt() performs exactly one attempted step (accepted or rejected) and
+// writes every output sample that step completes, so the kernels can run a
+// flat loop "while (!done) attempt()" in lock-step across a wavefront, and the
+// persistent kernel can refill a lane whose orbit has finished.
+//
+// The arithmetic follows the reference CPU integrators:
+//   DOP853: src/integrators/galpy/dop853.rs:316-783 (galpy leung_dop853.c)
+//   DOPR54: src/integrators/galpy/dopr54.rs:135-778 (galpy bovy_rk.c)
+// with these GPU-specific, value-preserving restatements:
+//   * DOP853 evaluates its three dense-output stages only on accepted steps
+//     that emit at least one sample (k10/k2/k3 are recomputed before reuse, so
+//     the trajectory is unchanged);
+//   * DOP853's err^(1/8) is sqrt(sqrt(sqrt(err))); pow(facold, beta) with
+//     beta = 0 is 1 and is dropped;
+//   * DOPR54's log-space error norm s = ln(e^(atol-c) + e^(rtol+max-c)) + c is
+//     the logarithm of sc = e^atol + e^rtol max|y|, so the norm is computed as
+//     mean((yerr/sc)^2), and round(log2(0.85 err^-0.2)) is decided by comparing
+//     that mean square against the six exact boundaries of the rounding.
+// Termination guards the reference lacks: a non-finite DOP853 error estimate or
+// max_attempts attempts end the orbit, flagging its status and writing NaN to
+// the samples not reached.
+#pragma once
+
+#include "drift_device.hpp"
+
+namespace drift {
+
+constexpr int32_t STATUS_NONFINITE = 0x100;
+constexpr int32_t STATUS_STEP_LIMIT = 0x200;
+
+// ---------------------------------------------------------------------------
+// DOP853 tableau (Hairer & Wanner; the coefficients galpy and the reference
+// use, src/integrators/galpy/dop853.rs:105-264).
+// ---------------------------------------------------------------------------
+namespace hw853 {
+constexpr double uround = 2.3e-16;
+constexpr double c2 = 0.526001519587677318785587544488e-1, c3 = 0.789002279381515978178381316732e-1,
+                 c4 = 0.118350341907227396726757197510, c5 = 0.281649658092772603273242802490,
+                 c6 = 0.333333333333333333333333333333, c7 = 0.25,
+                 c8 = 0.307692307692307692307692307692, c9 = 0.651282051282051282051282051282,
+                 c10 = 0.6, c11 = 0.857142857142857142857142857142, c14 = 0.1, c15 = 0.2,
+                 c16 = 0.777777777777777777777777777778;
+constexpr double a21 = 5.26001519587677318785587544488e-2;
+constexpr double a31 = 1.97250569845378994544595329183e-2, a32 = 5.91751709536136983633785987549e-2;
+constexpr double a41 = 2.95875854768068491816892993775e-2, a43 = 8.87627564304205475450678981324e-2;
+constexpr double a51 = 2.41365134159266685502369798665e-1, a53 = -8.84549479328286085344864962717e-1,
+                 a54 = 9.24834003261792003115737966543e-1;
+constexpr double a61 = 3.7037037037037037037037037037e-2, a64 = 1.70828608729473871279604482173e-1,
+                 a65 = 1.25467687566822425016691814123e-1;
+constexpr double a71 = 3.7109375e-2, a74 = 1.70252211019544039314978060272e-1,
+                 a75 = 6.02165389804559606850219397283e-2, a76 = -1.7578125e-2;
+constexpr double a81 = 3.70920001185047927108779319836e-2, a84 = 1.70383925712239993810214054705e-1,
+                 a85 = 1.07262030446373284651809199168e-1, a86 = -1.53194377486244017527936158236e-2,
+                 a87 = 8.27378916381402288758473766002e-3;
+constexpr double a91 = 6.24110958716075717114429577812e-1, a94 = -3.36089262944694129406857109825e0,
+                 a95 = -8.68219346841726006818189891453e-1, a96 = 2.75920996994467083049415600797e1,
+                 a97 = 2.01540675504778934086186788979e1, a98 = -4.34898841810699588477366255144e1;
+constexpr double a101 = 4.77662536438264365890433908527e-1, a104 = -2.48811461997166764192642586468e0,
+                 a105 = -5.90290826836842996371446475743e-1, a106 = 2.12300514481811942347288949897e1,
+                 a107 = 1.52792336328824235832596922938e1, a108 = -3.32882109689848629194453265587e1,
+                 a109 = -2.03312017085086261358222928593e-2;
+constexpr double a111 = -9.3714243008598732571704021658e-1, a114 = 5.18637242884406370830023853209e0,
+                 a115 = 1.09143734899672957818500254654e0, a116 = -8.14978701074692612513997267357e0,
+                 a117 = -1.85200656599969598641566180701e1, a118 = 2.27394870993505042818970056734e1,
+                 a119 = 2.49360555267965238987089396762e0, a1110 = -3.0467644718982195003823669022e0;
+constexpr double a121 = 2.27331014751653820792359768449e0, a124 = -1.05344954667372501984066689879e1,
+                 a125 = -2.00087205822486249909675718444e0, a126 = -1.79589318631187989172765950534e1,
+                 a127 = 2.79488845294199600508499808837e1, a128 = -2.85899827713502369474065508674e0,
+                 a129 = -8.87285693353062954433549289258e0, a1210 = 1.23605671757943030647266201528e1,
+                 a1211 = 6.43392746015763530355970484046e-1;
+constexpr double a141 = 5.61675022830479523392909219681e-2, a147 = 2.53500210216624811088794765333e-1,
+                 a148 = -2.46239037470802489917441475441e-1, a149 = -1.24191423263816360469010140626e-1,
+                 a1410 = 1.5329179827876569731206322685e-1, a1411 = 8.20105229563468988491666602057e-3,
+                 a1412 = 7.56789766054569976138603589584e-3, a1413 = -8.298e-3;
+constexpr double a151 = 3.18346481635021405060768473261e-2, a156 = 2.83009096723667755288322961402e-2,
+                 a157 = 5.35419883074385676223797384372e-2, a158 = -5.49237485713909884646569340306e-2,
+                 a1511 = -1.08347328697249322858509316994e-4, a1512 = 3.82571090835658412954920192323e-4,
+                 a1513 = -3.40465008687404560802977114492e-4, a1514 = 1.41312443674632500278074618366e-1;
+constexpr double a161 = -4.28896301583791923408573538692e-1, a166 = -4.69762141536116384314449447206e0,
+                 a167 = 7.68342119606259904184240953878e0, a168 = 4.06898981839711007970213554331e0,
+                 a169 = 3.56727187455281109270669543021e-1, a1613 = -1.39902416515901462129418009734e-3,
+                 a1614 = 2.9475147891527723389556272149e0, a1615 = -9.15095847217987001081870187138e0;
+constexpr double b1 = 5.42937341165687622380535766363e-2, b6 = 4.45031289275240888144113950566,
+                 b7 = 1.89151789931450038304281599044, b8 = -5.8012039600105847814672114227,
+                 b9 = 3.1116436695781989440891606237e-1, b10 = -1.52160949662516078556178806805e-1,
+                 b11 = 2.01365400804030348374776537501e-1, b12 = 4.47106157277725905176885569043e-2;
+constexpr double bhh1 = 0.244094488188976377952755905512, bhh2 = 0.733846688281611857341361741547,
+                 bhh3 = 0.220588235294117647058823529412e-1;
+constexpr double er1 = 0.1312004499419488073250102996e-1, er6 = -0.1225156446376204440720569753e1,
+                 er7 = -0.4957589496572501915214079952, er8 = 0.1664377182454986536961530415e1,
+                 er9 = -0.3503288487499736816886487290, er10 = 0.3341791187130174790297318841,
+                 er11 = 0.8192320648511571246570742613e-1, er12 = -0.2235530786388629525884427845e-1;
+constexpr double d41 = -0.84289382761090128651353491142e1, d46 = 0.56671495351937776962531783590,
+                 d47 = -0.30689499459498916912797304727e1, d48 = 0.23846676565120698287728149680e1,
+                 d49 = 0.21170345824450282767155149946e1, d410 = -0.87139158377797299206789907490,
+                 d411 = 0.22404374302607882758541771650e1, d412 = 0.63157877876946881815570249290,
+                 d413 = -0.88990336451333310820698117400e-1, d414 = 0.18148505520854727256656404962e2,
+                 d415 = -0.91946323924783554000451984436e1, d416 = -0.44360363875948939664310572000e1;
+constexpr double d51 = 0.10427508642579134603413151009e2, d56 = 0.24228349177525818288430175319e3,
+                 d57 = 0.16520045171727028198505394887e3, d58 = -0.37454675472269020279518312152e3,
+                 d59 = -0.22113666853125306036270938578e2, d510 = 0.77334326684722638389603898808e1,
+                 d511 = -0.30674084731089398182061213626e2, d512 = -0.93321305264302278729567221706e1,
+                 d513 = 0.15697238121770843886131091075e2, d514 = -0.31139403219565177677282850411e2,
+                 d515 = -0.93529243588444783865713862664e1, d516 = 0.35816841486394083752465898540e2;
+constexpr double d61 = 0.19985053242002433820987653617e2, d66 = -0.38703730874935176555105901742e3,
+                 d67 = -0.18917813819516756882830838328e3, d68 = 0.52780815920542364900561016686e3,
+                 d69 = -0.11573902539959630126141871134e2, d610 = 0.68812326946963000169666922661e1,
+                 d611 = -0.10006050966910838403183860980e1, d612 = 0.77771377980534432092869265740,
+                 d613 = -0.27782057523535084065932004339e1, d614 = -0.60196695231264120758267380846e2,
+                 d615 = 0.84320405506677161018159903784e2, d616 = 0.11992291136182789328035130030e2;
+constexpr double d71 = -0.25693933462703749003312586129e2, d76 = -0.15418974869023643374053993627e3,
+                 d77 = -0.23152937917604549567536039109e3, d78 = 0.35763911791061412378285349910e3,
+                 d79 = 0.93405324183624310003907691704e2, d710 = -0.37458323136451633156875139351e2,
+                 d711 = 0.10409964950896230045147246184e3, d712 = 0.29840293426660503123344363579e2,
+                 d713 = -0.43533456590011143754432175058e2, d714 = 0.96324553959188282948394950600e2,
+                 d715 = -0.39177261675615439165231486172e2, d716 = -0.14972683625798562581422125276e3;
+// step-size controller (dop853.rs:342-347); beta = 0 so facold drops out
+constexpr double safe = 0.9, facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;
+}  // namespace hw853
+
+// The reference's ternary min/max (dop853.rs:286-299): NaN handling differs
+// from fmin/fmax and is kept.
+DRIFT_HD DRIFT_FI double tmin(double a, double b) { return a < b ? a : b; }
+DRIFT_HD DRIFT_FI double tmax(double a, double b) { return a > b ? a : b; }
+DRIFT_HD DRIFT_FI double sign_of(double x, double y) { return y > 0.0 ? std::fabs(x) : -std::fabs(x); }
+
+DRIFT_HD DRIFT_FI bool finite6(const double* y) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) ok = ok && std::isfinite(y[i]);
+    return ok;
+}
+
+// Uniform (per-job) DOP853 data.
+struct Dop853Job {
+    const double* times;
+    int32_t nt;
+    uint32_t max_attempts;
+    double rtol, atol;  // linear tolerances (exp of the ABI's logarithms)
+    double hmax, pos_neg;
+};
+
+struct Dop853Lane {
+    double y[DIM], k1[DIM];
+    double h, t_cur, t_old;
+    int32_t ii;  // index of the last emitted sample
+    int32_t reject;
+    uint32_t accepted, rejected, dense;
+    int32_t status;
+};
+
+DRIFT_HD DRIFT_FI Dop853Job make_dop853_job(const double* times, int32_t nt, double rtol_lin,
+                                             double atol_lin, uint32_t max_attempts, double t0,
+                                             double tend) {
+    Dop853Job j;
+    j.times = times;
+    j.nt = nt;
+    j.max_attempts = max_attempts;
+    j.rtol = rtol_lin;
+    j.atol = atol_lin;
+    j.hmax = tend - t0;                 // dop853.rs:348
+    j.pos_neg = sign_of(1.0, j.hmax);   // dop853.rs:349
+    return j;
+}
+
+// Initial step selection (Hairer), dop853.rs:372-423.  Emits sample 0.
+template <class F, class Sink>
+DRIFT_HD DRIFT_FI void dop853_init(Dop853Lane& L, const double* y0, double t0, const Dop853Job& J,
+                                   const F& f, Sink& sink) {
+#pragma unroll
+    for (int i = 0; i < DIM; i++) L.y[i] = y0[i];
+    sink.emit(0, L.y);
+    f(L.y, L.k1);
+    double dnf = 0.0, dny = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        double sk = J.atol + J.rtol * std::fabs(L.y[i]);
+        double sqr = L.k1[i] / sk;
+        dnf += sqr * sqr;
+        sqr = L.y[i] / sk;
+        dny += sqr * sqr;
+    }
+    double h = sign_of(tmin(std::sqrt(dny / dnf) * 0.01, std::fabs(J.hmax)), J.pos_neg);
+    double k3[DIM], k2[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++) k3[i] = L.y[i] + h * L.k1[i];
+    f(k3, k2);
+    double der2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        double sk = J.atol + J.rtol * std::fabs(L.y[i]);
+        double sqr = (k2[i] - L.k1[i]) / sk;
+        der2 += sqr * sqr;
+    }
+    der2 = std::sqrt(der2) / h;
+    double der12 = tmax(std::fabs(der2), std::sqrt(dnf));
+    double h1 = std::sqrt(std::sqrt(std::sqrt(0.01 / der12)));  // (0.01/der12)^(1/8)
+    L.h = sign_of(tmin(100.0 * std::fabs(h), tmin(std::fabs(h1), std::fabs(J.hmax))), J.pos_neg);
+    L.t_cur = t0;
+    L.t_old = t0;
+    L.ii = 0;
+    L.reject = 0;
+    L.accepted = L.rejected = L.dense = 0;
+    L.status = 0;
+}
+
+// Writes NaN to the samples not reached and marks the lane finished.
+template <class Sink>
+DRIFT_HD DRIFT_FI void abandon_orbit(int32_t& ii, int32_t nt, int32_t& status, int32_t flag,
+                                     Sink& sink) {
+    double nan6[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++) nan6[i] = NAN;
+    while (ii < nt - 1) {
+        ii++;
+        sink.emit(ii, nan6);
+    }
+    status |= flag;
+}
+
+// One attempted DOP853 step, dop853.rs:425-782.  Returns true when the
+// orbit has emitted its last sample.
+template <class F, class Sink>
+DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F& f, Sink& sink) {
+    using namespace hw853;
+    const double h = J.pos_neg * tmax(std::fabs(L.h), 1e3 * uround);
+    const double* y0 = L.y;
+    const double* k1 = L.k1;
+    double yy1[DIM], k2[DIM], k3[DIM], k4[DIM], k5[DIM], k6[DIM], k7[DIM], k8[DIM], k9[DIM],
+        k10[DIM];
+
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * a21 * k1[i];
+    f(yy1, k2);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * (a31 * k1[i] + a32 * k2[i]);
+    f(yy1, k3);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * (a41 * k1[i] + a43 * k3[i]);
+    f(yy1, k4);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * (a51 * k1[i] + a53 * k3[i] + a54 * k4[i]);
+    f(yy1, k5);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * (a61 * k1[i] + a64 * k4[i] + a65 * k5[i]);
+    f(yy1, k6);
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+        yy1[i] = y0[i] + h * (a71 * k1[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]);
+    f(yy1, k7);
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+        yy1[i] = y0[i] + h * (a81 * k1[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] + a87 * k7[i]);
+    f(yy1, k8);
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+        yy1[i] = y0[i] + h * (a91 * k1[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] + a97 * k7[i] +
+                              a98 * k8[i]);
+    f(yy1, k9);
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+        yy1[i] = y0[i] + h * (a101 * k1[i] + a104 * k4[i] + a105 * k5[i] + a106 * k6[i] +
+                              a107 * k7[i] + a108 * k8[i] + a109 * k9[i]);
+    f(yy1, k10);
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+        yy1[i] = y0[i] + h * (a111 * k1[i] + a114 * k4[i] + a115 * k5[i] + a116 * k6[i] +
+                              a117 * k7[i] + a118 * k8[i] + a119 * k9[i] + a1110 * k10[i]);
+    f(yy1, k2);
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+        yy1[i] = y0[i] + h * (a121 * k1[i] + a124 * k4[i] + a125 * k5[i] + a126 * k6[i] +
+                              a127 * k7[i] + a128 * k8[i] + a129 * k9[i] + a1210 * k10[i] +
+                              a1211 * k2[i]);
+    const double t_old_older = L.t_old;
+    const double t_step = L.t_cur;  // start of this step
+    const double t_new = L.t_cur + h;
+    f(yy1, k3);
+
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        k4[i] = b1 * k1[i] + b6 * k6[i] + b7 * k7[i] + b8 * k8[i] + b9 * k9[i] + b10 * k10[i] +
+                b11 * k2[i] + b12 * k3[i];
+        k5[i] = y0[i] + h * k4[i];
+    }
+    double err = 0.0, err2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        double sk = J.atol + J.rtol * tmax(std::fabs(y0[i]), std::fabs(k5[i]));
+        double erri = k4[i] - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i];
+        double sqr = erri / sk;
+        err2 += sqr * sqr;
+        erri = er1 * k1[i] + er6 * k6[i] + er7 * k7[i] + er8 * k8[i] + er9 * k9[i] +
+               er10 * k10[i] + er11 * k2[i] + er12 * k3[i];
+        sqr = erri / sk;
+        err += sqr * sqr;
+    }
+    double deno = err + 0.01 * err2;
+    if (deno <= 0.0) deno = 1.0;
+    err = std::fabs(h) * err * std::sqrt(1.0 / (deno * (double)DIM));
+
+    if (err != err) {
+        // The reference would shrink h to its floor and retry forever.
+        abandon_orbit(L.ii, J.nt, L.status, STATUS_NONFINITE, sink);
+        return true;
+    }
+    const double fac11 = std::sqrt(std::sqrt(std::sqrt(err)));  // err^(1/8)
+    const double fac = tmax(facc2, tmin(facc1, fac11 / safe));
+    double hnew = h / fac;
+
+    if (err <= 1.0) {
+        L.accepted++;
+        f(k5, k4);  // FSAL derivative at the new point
+        const double* times = J.times;
+        const int32_t last = J.nt - 1;
+        if (L.ii < last && J.pos_neg * times[L.ii + 1] < J.pos_neg * t_new) {
+            // Dense output (dop853.rs:636-763), only when a sample falls in
+            // (t_step, t_new).
+            L.dense++;
+            double r1[DIM], r2[DIM], r3[DIM], r4[DIM], r5[DIM], r6[DIM], r7[DIM], r8[DIM];
+#pragma unroll
+            for (int i = 0; i < DIM; i++) {
+                r1[i] = y0[i];
+                double ydiff = k5[i] - y0[i];
+                r2[i] = ydiff;
+                double bspl = h * k1[i] - ydiff;
+                r3[i] = bspl;
+                r4[i] = ydiff - h * k4[i] - bspl;
+                r5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
+                        d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
+                r6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
+                        d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
+                r7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
+                        d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
+                r8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
+                        d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
+            }
+#pragma unroll
+            for (int i = 0; i < DIM; i++)
+                yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
+                                      a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] +
+                                      a1413 * k4[i]);
+            f(yy1, k10);
+#pragma unroll
+            for (int i = 0; i < DIM; i++)
+                yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
+                                      a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] +
+                                      a1514 * k10[i]);
+            f(yy1, k2);
+#pragma unroll
+            for (int i = 0; i < DIM; i++)
+                yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
+                                      a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] +
+                                      a1615 * k2[i]);
+            f(yy1, k3);
+#pragma unroll
+            for (int i = 0; i < DIM; i++) {
+                r5[i] = h * (r5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
+                r6[i] = h * (r6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
+                r7[i] = h * (r7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
+                r8[i] = h * (r8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
+            }
+            do {
+                const double s = (times[L.ii + 1] - t_step) / h;
+                const double s1 = 1.0 - s;
+                double yy[DIM];
+#pragma unroll
+                for (int i = 0; i < DIM; i++)
+                    yy[i] = r1[i] +
+                            s * (r2[i] +
+                                 s1 * (r3[i] +
+                                       s * (r4[i] + s1 * (r5[i] + s * (r6[i] + s1 * (r7[i] + s * r8[i]))))));
+                L.ii++;
+                if (!finite6(yy)) L.status |= STATUS_NONFINITE;
+                sink.emit(L.ii, yy);
+            } while (L.ii < last && J.pos_neg * times[L.ii + 1] < J.pos_neg * t_new);
+        }
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            L.k1[i] = k4[i];
+            L.y[i] = k5[i];
+        }
+        hnew = std::fabs(hnew) > std::fabs(J.hmax) ? J.pos_neg * J.hmax : hnew;
+        if (L.reject != 0) hnew = J.pos_neg * tmin(std::fabs(hnew), std::fabs(h));
+        L.reject = 0;
+        L.t_old = t_step;
+        L.t_cur = t_new;
+    } else {
+        L.rejected++;
+        hnew = h / tmin(facc1, fac11 / safe);
+        L.reject = 1;
+        // t_current = t_old; t_old = t_old_older (dop853.rs:777-778)
+        L.t_cur = t_step;
+        L.t_old = t_old_older;
+    }
+    L.h = hnew;
+    if (L.ii >= J.nt - 1) return true;
+    if (L.accepted + L.rejected >= J.max_attempts) {
+        abandon_orbit(L.ii, J.nt, L.status, STATUS_STEP_LIMIT, sink);
+        return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// DOPR54 (galpy bovy_rk.c), dopr54.rs.
+// ---------------------------------------------------------------------------
+namespace dp54 {
+constexpr double c2 = 0.2, c3 = 0.3, c4 = 0.8, c5 = 8.0 / 9.0;
+constexpr double a21 = 0.2;
+constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
+constexpr double a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
+constexpr double a51 = 19372.0 / 6561.0, a52 = -25360.0 / 2187.0, a53 = 64448.0 / 6561.0,
+                 a54 = -212.0 / 729.0;
+constexpr double a61 = 9017.0 / 3168.0, a62 = -355.0 / 33.0, a63 = 46732.0 / 5247.0,
+                 a64 = 49.0 / 176.0, a65 = -5103.0 / 18656.0;
+constexpr double a71 = 35.0 / 384.0, a73 = 500.0 / 1113.0, a74 = 125.0 / 192.0,
+                 a75 = -2187.0 / 6784.0, a76 = 11.0 / 84.0;
+constexpr double b1 = 35.0 / 384.0, b3 = 500.0 / 1113.0, b4 = 125.0 / 192.0, b5 = -2187.0 / 6784.0,
+                 b6 = 11.0 / 84.0;
+constexpr double e1 = b1 - 5179.0 / 57600.0, e3 = b3 - 7571.0 / 16695.0, e4 = b4 - 393.0 / 640.0,
+                 e5 = b5 + 92097.0 / 339200.0, e6 = b6 - 187.0 / 2100.0, e7 = -1.0 / 40.0;
+constexpr double max_stepreduce = 10000.0;  // dopr54.rs:34
+constexpr double max_dt_reduce = 10000.0;   // dopr54.rs:35
+constexpr double initial_step_sentinel = -9999.99;
+}  // namespace dp54
+
+struct Dopr54Job {
+    const double* times;
+    int32_t nt;
+    uint32_t max_attempts;
+    double dt;             // t[1] - t[0], the only grid spacing DOPR54 reads (dopr54.rs:713)
+    double rtol, atol;     // linear tolerances e^rtol, e^atol
+    double dt_one_init;    // sentinel selects the RK4 estimate
+    double bound2[6];      // squared error-norm boundaries of round(log2(0.85 err^-0.2)) >= j, j = -2..3
+};
+
+struct Dopr54Lane {
+    double y[DIM], a1[DIM];
+    double to, dt_one, init_to, init_dt_one;
+    int32_t ii;
+    int32_t err;  // reference error code (dopr54.rs:407-418)
+    uint32_t accepted, rejected;
+    int32_t status;
+};
+
+// Boundaries err <= U_j  <=>  round(log2(0.85) - 0.2 log2(err)) >= j, with
+// U_j = 0.85^5 2^(5 (1/2 - j)); stored squared for comparison with mean(e^2).
+inline void dopr54_bounds(double* bound2) {
+    for (int j = -2; j <= 3; ++j) {
+        double u = std::pow(0.85, 5.0) * std::ldexp(std::sqrt(2.0), 2 - 5 * j);  // 2^(2.5-5j)
+        bound2[j + 2] = u * u;
+    }
+}
+
+// rk4_onestep, dopr54.rs:135-208.
+template <class F>
+DRIFT_HD DRIFT_FI void rk4_onestep(const double* yn, double* yn1, double dt, double* ynk, double* a,
+                                   const F& f) {
+    f(yn, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 6.0;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) ynk[i] = yn[i] + dt * a[i] / 2.0;
+    f(ynk, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 3.0;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) ynk[i] = yn[i] + dt * a[i] / 2.0;
+    f(ynk, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 3.0;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) ynk[i] = yn[i] + dt * a[i];
+    f(ynk, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 6.0;
+}
+
+DRIFT_HD DRIFT_FI double max_abs6(const double* y) {
+    double m = std::fabs(y[0]);
+#pragma unroll
+    for (int i = 1; i < DIM; i++) m = tmax(std::fabs(y[i]), m);
+    return m;
+}
+
+// rk4_estimate_step, dopr54.rs:210-368, with the scale e^s = atol + rtol max|y|.
+template <class F>
+DRIFT_HD DRIFT_FI double rk4_estimate_step(const double* yo, double dt, const Dopr54Job& J,
+                                           const F& f) {
+    const double init_dt = dt;
+    const double sc = J.atol + J.rtol * max_abs6(yo);
+    double yn[DIM], y1[DIM], y21[DIM], y2[DIM], ynk[DIM], a[DIM];
+    for (int iter = 0; iter < 64; ++iter) {
+#pragma unroll
+        for (int i = 0; i < DIM; i++) yn[i] = y1[i] = y21[i] = yo[i];
+        rk4_onestep(yn, y1, dt, ynk, a, f);
+        rk4_onestep(yn, y21, dt / 2.0, ynk, a, f);
+#pragma unroll
+        for (int i = 0; i < DIM; i++) y2[i] = y21[i];
+        rk4_onestep(y21, y2, dt / 2.0, ynk, a, f);
+        double e = 0.0;
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            double q = (y1[i] - y2[i]) / sc;
+            e += q * q;
+        }
+        double err = std::sqrt(e / (double)DIM);
+        if (!(err > 1.0)) break;
+        double factor = std::ceil(std::pow(err, 0.2));
+        if (factor > 1.0 && init_dt / dt * factor < dp54::max_dt_reduce) {
+            dt /= factor;
+        } else {
+            break;
+        }
+    }
+    return dt;
+}
+
+// Writes samples for every interval the lane has completed (the save after
+// each dopr54_onestep, dopr54.rs:728-764) and opens the next interval.
+template <class Sink>
+DRIFT_HD DRIFT_FI void dopr54_emit_completed(Dopr54Lane& L, const Dopr54Job& J, Sink& sink) {
+    const double dt = J.dt;
+    while (L.ii < J.nt - 1) {
+        const double end = L.init_to + dt;
+        const bool open = (dt >= 0.0 && L.to < end) || (dt < 0.0 && L.to > end);
+        if (open) break;
+        L.ii++;
+        if (!finite6(L.y)) L.status |= STATUS_NONFINITE;
+        sink.emit(L.ii, L.y);
+        L.init_to = L.to;
+        L.init_dt_one = L.dt_one;
+    }
+}
+
+template <class F, class Sink>
+DRIFT_HD DRIFT_FI void dopr54_init(Dopr54Lane& L, const double* y0, double t0, const Dopr54Job& J,
+                                   const F& f, Sink& sink) {
+#pragma unroll
+    for (int i = 0; i < DIM; i++) L.y[i] = y0[i];
+    sink.emit(0, L.y);
+    double dt_one = J.dt_one_init;
+    if (dt_one == dp54::initial_step_sentinel) dt_one = rk4_estimate_step(L.y, J.dt, J, f);
+    L.to = t0;
+    f(L.y, L.a1);
+    L.dt_one = dt_one;
+    L.init_to = t0;
+    L.init_dt_one = dt_one;
+    L.ii = 0;
+    L.err = 0;
+    L.accepted = L.rejected = 0;
+    L.status = 0;
+    dopr54_emit_completed(L, J, sink);
+}
+
+// One pass of the dopr54_onestep loop body (dopr54.rs:403-459) with
+// dopr54_actualstep (:462-659).  Returns true when the orbit is complete.
+template <class F, class Sink>
+DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F& f, Sink& sink) {
+    using namespace dp54;
+    const double dt = J.dt;
+    bool accept = false;
+    double dt_one = L.dt_one;
+    if (L.init_dt_one / dt_one > max_stepreduce || dt_one != dt_one) {
+        dt_one = L.init_dt_one / max_stepreduce;
+        accept = true;
+        if (L.err % 2 == 0) L.err += 1;
+    }
+    if (dt >= 0.0 && dt_one > (L.init_to + dt - L.to)) dt_one = L.init_to + dt - L.to;
+    if (dt < 0.0 && dt_one < (L.init_to + dt - L.to)) dt_one = L.init_to + dt - L.to;
+
+    const double* yo = L.y;
+    double yn1[DIM], yerr[DIM], ynk[DIM], a[DIM], k1[DIM], k2[DIM], k3[DIM], k4[DIM], k5[DIM],
+        k6[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        k1[i] = dt_one * L.a1[i];
+        yn1[i] = yo[i] + b1 * k1[i];
+        yerr[i] = e1 * k1[i];
+        ynk[i] = yo[i] + a21 * k1[i];
+    }
+    f(ynk, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        k2[i] = dt_one * a[i];
+        ynk[i] = yo[i] + a31 * k1[i] + a32 * k2[i];
+    }
+    f(ynk, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        k3[i] = dt_one * a[i];
+        yn1[i] += b3 * k3[i];
+        yerr[i] += e3 * k3[i];
+        ynk[i] = yo[i] + a41 * k1[i] + a42 * k2[i] + a43 * k3[i];
+    }
+    f(ynk, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        k4[i] = dt_one * a[i];
+        yn1[i] += b4 * k4[i];
+        yerr[i] += e4 * k4[i];
+        ynk[i] = yo[i] + a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i];
+    }
+    f(ynk, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        k5[i] = dt_one * a[i];
+        yn1[i] += b5 * k5[i];
+        yerr[i] += e5 * k5[i];
+        ynk[i] = yo[i] + a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i];
+    }
+    f(ynk, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        k6[i] = dt_one * a[i];
+        yn1[i] += b6 * k6[i];
+        yerr[i] += e6 * k6[i];
+        ynk[i] = yo[i] + a71 * k1[i] + a73 * k3[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i];
+    }
+    f(ynk, a);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) yerr[i] += e7 * dt_one * a[i];
+
+    // error norm: mean((yerr / (atol + rtol max|yo|))^2)
+    const double maxabs = max_abs6(yo);
+    const double sc = J.atol + J.rtol * maxabs;
+    double e = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        double q = yerr[i] / sc;
+        e += q * q;
+    }
+    e = e / (double)DIM;
+    if (!std::isfinite(maxabs)) e = NAN;
+    double powertwo;
+    if (e != e) {
+        powertwo = NAN;
+    } else {
+        int p = -3;
+#pragma unroll
+        for (int j = 0; j < 6; j++) p += (e <= J.bound2[j]) ? 1 : 0;
+        powertwo = (double)p;
+    }
+    if (powertwo >= 0.0 || accept) {
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            L.a1[i] = a[i];
+            L.y[i] = yn1[i];
+        }
+        L.to += dt_one;
+        L.accepted++;
+    } else {
+        L.rejected++;
+    }
+    L.dt_one = (powertwo != powertwo) ? NAN : std::ldexp(dt_one, (int)powertwo);
+    dopr54_emit_completed(L, J, sink);
+    if (L.ii >= J.nt - 1) return true;
+    if (L.accepted + L.rejected >= J.max_attempts) {
+        abandon_orbit(L.ii, J.nt, L.status, STATUS_STEP_LIMIT, sink);
+        return true;
+    }
+    return false;
+}
+
+}  // namespace drift

@@ -1,0 +1,201 @@
+// spherical_table.cpp -- host-side construction of the Milky-Way spherical
+// force table used by the gfx950 kernels.
+//
+// The bulge (cut-off power law) and NFW halo are both spherical, so their
+// summed acceleration is -S(r^2) r_vec with
+//   S(r2) = M_b(r) / r^3 + amp_h (ln(1 + r/a_h) - r/(a_h + r)) / r^3,
+//   M_b(r) = 2 pi amp_b rc^(3-alpha) gamma(3/2 - alpha/2, r^2/rc^2).
+// S has its only singularity at r2 = 0, so on every half-octave of r2 it is
+// approximated to ~1e-16 by a Chebyshev interpolant, evaluated in long double
+// here and stored as monomial coefficients in the segment's local variable.
+// The kernels then need no log, exp or pow for these two components.
+//
+// gamma(a, x) is evaluated by its power series for x < a + 1 and by the Lentz
+// continued fraction of Gamma(a, x) otherwise (Abramowitz & Stegun 6.5.29 /
+// 6.5.31; Press et al., Numerical Recipes 2nd ed. section 6.2), in long double.
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "drift_device.hpp"
+#include "drift_internal.hpp"
+
+namespace drift {
+
+namespace {
+
+using ld = long double;
+
+ld lower_gamma_ld(ld a, ld x) {
+    if (x <= 0) return 0;
+    if (x < a + 1) {
+        ld ap = a, del = 1 / a, sum = del;
+        for (int n = 0; n < 2000; ++n) {
+            ap += 1;
+            del *= x / ap;
+            sum += del;
+            if (fabsl(del) < fabsl(sum) * 1e-21L) break;
+        }
+        return sum * expl(-x + a * logl(x));
+    }
+    const ld fpmin = 1e-4000L;
+    ld bb = x + 1 - a, c = 1 / fpmin, d = 1 / bb, h = d;
+    for (int i = 1; i < 2000; ++i) {
+        ld an = -i * (i - a);
+        bb += 2;
+        d = an * d + bb;
+        if (fabsl(d) < fpmin) d = fpmin;
+        c = bb + an / c;
+        if (fabsl(c) < fpmin) c = fpmin;
+        d = 1 / d;
+        ld del = d * c;
+        h *= del;
+        if (fabsl(del - 1) < 1e-21L) break;
+    }
+    return tgammal(a) - expl(-x + a * logl(x)) * h;
+}
+
+// ln(1+u) - u/(1+u), with the alternating series for small u where the
+// direct difference cancels.
+ld nfw_mass_shape(ld u) {
+    if (u < 0.125L) {
+        // sum_{k>=2} (-1)^k (k-1)/k u^k
+        ld pw = u, sum = 0;
+        for (int k = 2; k < 80; ++k) {
+            pw *= u;
+            ld add = (k % 2 == 0 ? 1 : -1) * (ld)(k - 1) / (ld)k * pw;
+            sum += add;
+            if (fabsl(add) < 1e-24L * fabsl(sum)) break;
+        }
+        return sum;
+    }
+    return log1pl(u) - u / (1 + u);
+}
+
+struct MwSpherical {
+    ld mass_scale, a_bulge, rc2, halo_amp, halo_a;
+    ld operator()(ld r2) const {
+        ld r = sqrtl(r2);
+        ld r3 = r2 * r;
+        ld sb = mass_scale * lower_gamma_ld(a_bulge, r2 / rc2) / r3;
+        ld sh = halo_amp * nfw_mass_shape(r / halo_a) / r3;
+        return sb + sh;
+    }
+};
+
+MwSpherical make_spherical(const drift_potential& p) {
+    MwSpherical f;
+    f.a_bulge = 1.5L - 0.5L * (ld)p.bulge_alpha;
+    f.mass_scale = 2 * 3.14159265358979323846264338327950288L * (ld)p.bulge_amp *
+                   powl((ld)p.bulge_rc, 3 - (ld)p.bulge_alpha);
+    f.rc2 = (ld)p.bulge_rc * (ld)p.bulge_rc;
+    f.halo_amp = p.halo_amp;
+    f.halo_a = p.halo_a;
+    return f;
+}
+
+void segment_bounds(int seg, ld& lo, ld& hi, int& e) {
+    e = seg / 2 + TABLE_E_MIN;
+    const double kSqrt2 = 1.4142135623730951;
+    const int half = seg % 2;
+    lo = half ? (ld)kSqrt2 : 1.0L;
+    hi = half ? 2.0L : (ld)kSqrt2;
+}
+
+}  // namespace
+
+void build_mw_params(const drift_potential& p, PotParams& out) {
+    out = PotParams{};
+    out.kind = POT_MW;
+    out.has_table = 1;
+    out.disk_amp = p.disk_amp;
+    out.disk_a = p.disk_a;
+    out.disk_b2 = p.disk_b * p.disk_b;
+    const long double a = 1.5L - 0.5L * (long double)p.bulge_alpha;
+    const long double mass_scale = 2 * 3.14159265358979323846264338327950288L *
+                                   (long double)p.bulge_amp *
+                                   powl((long double)p.bulge_rc, 3 - (long double)p.bulge_alpha);
+    const long double rc2 = (long double)p.bulge_rc * (long double)p.bulge_rc;
+    out.bulge_mass_total = (double)(mass_scale * tgammal(a));
+    // gamma(a, x) = x^a (1/a - x/(a+1) + ...), x = r2/rc2:
+    // S_b = mass_scale rc2^-a r2^(a - 3/2) (1/a - r2/(rc2 (a+1))).
+    const long double pre = mass_scale * powl(rc2, -a);
+    out.bulge_small_c0 = (double)(pre / a);
+    out.bulge_small_c1 = (double)(-pre / (rc2 * (a + 1)));
+    out.bulge_exponent = (double)(a - 1.5L);
+    out.halo_amp = p.halo_amp;
+    out.halo_a = p.halo_a;
+}
+
+void build_mw_table(const drift_potential& p, double* coeffs) {
+    const MwSpherical f = make_spherical(p);
+    const int N = TABLE_NCOEF;
+    const ld pi = 3.14159265358979323846264338327950288L;
+    for (int seg = 0; seg < TABLE_SEGMENTS; ++seg) {
+        ld lo, hi;
+        int e;
+        segment_bounds(seg, lo, hi, e);
+        const ld scale = ldexpl(1.0L, e);
+        const ld centre = 0.5L * (lo + hi), half = 0.5L * (hi - lo);
+        // Chebyshev interpolation at the N first-kind nodes.
+        ld fv[TABLE_NCOEF], cheb[TABLE_NCOEF];
+        for (int j = 0; j < N; ++j) {
+            ld th = pi * (j + 0.5L) / N;
+            ld s = cosl(th);
+            fv[j] = f((centre + half * s) * scale);
+        }
+        for (int k = 0; k < N; ++k) {
+            ld acc = 0;
+            for (int j = 0; j < N; ++j) acc += fv[j] * cosl(k * pi * (j + 0.5L) / N);
+            cheb[k] = 2 * acc / N;
+        }
+        cheb[0] *= 0.5L;
+        // Convert sum_k cheb[k] T_k(s) to monomials in s.
+        ld mono[TABLE_NCOEF] = {};
+        ld tkm1[TABLE_NCOEF] = {}, tk[TABLE_NCOEF] = {}, tkp1[TABLE_NCOEF];
+        tkm1[0] = 1;  // T_0
+        tk[1] = 1;    // T_1
+        mono[0] += cheb[0];
+        if (N > 1) mono[1] += cheb[1];
+        for (int k = 2; k < N; ++k) {
+            for (int i = 0; i < N; ++i) tkp1[i] = -tkm1[i] + (i > 0 ? 2 * tk[i - 1] : 0);
+            for (int i = 0; i < N; ++i) mono[i] += cheb[k] * tkp1[i];
+            for (int i = 0; i < N; ++i) {
+                tkm1[i] = tk[i];
+                tk[i] = tkp1[i];
+            }
+        }
+        for (int i = 0; i < N; ++i) coeffs[seg * N + i] = (double)mono[i];
+    }
+}
+
+double mw_table_max_rel_error(const drift_potential& p, const double* coeffs, int samples_per_seg) {
+    const MwSpherical f = make_spherical(p);
+    double worst = 0.0;
+    uint64_t state = 0x9E3779B97F4A7C15ull;
+    for (int seg = 0; seg < TABLE_SEGMENTS; ++seg) {
+        ld lo, hi;
+        int e;
+        segment_bounds(seg, lo, hi, e);
+        for (int k = 0; k < samples_per_seg; ++k) {
+            state = state * 6364136223846793005ull + 1442695040888963407ull;
+            double u = (double)(state >> 11) * (1.0 / 9007199254740992.0);
+            double m = (double)(lo + (hi - lo) * (ld)u);
+            double r2 = std::ldexp(m, e);
+            int sg;
+            double s;
+            if (!table_locate(r2, sg, s)) continue;
+            double approx = horner_table(coeffs + sg * TABLE_NCOEF, s);
+            ld exact = f((ld)r2);
+            double rel = (double)fabsl(((ld)approx - exact) / exact);
+            if (rel > worst) worst = rel;
+        }
+    }
+    return worst;
+}
+
+double mw_spherical_exact(const drift_potential& p, double r2) {
+    return (double)make_spherical(p)((long double)r2);
+}
+
+}  // namespace drift

@@ -1,0 +1,179 @@
+/*
+ * drift_hip.h -- C ABI of the MI355X (gfx950) batched Dormand-Prince integrator.
+ *
+ * Drop-in replacement for the CUDA backend of drift's GPU engine.  The entry
+ * points carry the reference kernel ABI (kernels/src/lib.rs:143-216):
+ *   state0     n x 6 doubles, AoS rows (x, y, z, vx, vy, vz)
+ *   times      nt doubles (the output grid, src/dispatch/mod.rs:14-18)
+ *   state_out  nt x n x 6 doubles, time-major (kernels/src/lib.rs:20-33)
+ *   rtol, atol NATURAL LOGARITHMS of the tolerances, as galpy takes them
+ *              (src/integrators/galpy/mod.rs:17-24)
+ *   dt_one_init  DOPR54 initial sub-step; -9999.99 selects the RK4
+ *              step-doubling estimate (src/integrators/galpy/mod.rs:8)
+ * and the launch contract of the reference (n >= 1, nt >= 2; the reference's
+ * nt <= 1024 limit is lifted).  Violations return DRIFT_ERR_INVALID_ARGUMENT
+ * with a message from drift_last_error(), never a device fault.
+ *
+ * Additions over the reference ABI (none changes the meaning of the above):
+ *   - a potential descriptor (NULL = the reference's hard-coded unit galpy
+ *     Kepler force, src/dispatch/gpu.rs:155-156);
+ *   - per-orbit statistics and status (the reference GPU has no error channel);
+ *   - a session handle owning the device, stream and cached buffers
+ *     (the reference creates a context per launch, gpu/cuda_oxide.rs:16-18);
+ *   - device-resident variants that take device pointers and a HIP stream.
+ *
+ * All functions return DRIFT_OK (0) or a drift_status code.  Thread safety: a
+ * session may be used from one thread at a time; distinct sessions are
+ * independent.
+ */
+#ifndef DRIFT_HIP_H
+#define DRIFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRIFT_ABI_VERSION 1
+
+/* Status codes.  They map onto the reference's error taxonomy
+ * (DispatchError, src/dispatch/mod.rs:36-79; IntegrationError,
+ * src/integrators/mod.rs:10-20). */
+typedef enum drift_status {
+    DRIFT_OK = 0,
+    DRIFT_ERR_INVALID_ARGUMENT = 1, /* launch contract violated (kernels/src/lib.rs:129-152) */
+    DRIFT_ERR_UNSUPPORTED = 2,      /* UnsupportedConfiguration analogue */
+    DRIFT_ERR_DEVICE = 3,           /* DispatchError::Cuda analogue (HIP runtime error) */
+    DRIFT_ERR_INTEGRATION = 4,      /* an orbit returned a non-zero DOPR54 error code
+                                       (DispatchError::CpuIntegration analogue) */
+    DRIFT_ERR_OUT_OF_MEMORY = 5
+} drift_status;
+
+typedef enum drift_method {
+    DRIFT_METHOD_DOPR54 = 0, /* galpy bovy_rk.c DOPR54, src/integrators/galpy/dopr54.rs */
+    DRIFT_METHOD_DOP853 = 1  /* galpy leung_dop853.c DOP853, src/integrators/galpy/dop853.rs */
+} drift_method;
+
+typedef enum drift_potential_kind {
+    DRIFT_POTENTIAL_GALPY_KEPLER = 0, /* unit point mass, galpy cylindrical form (dopr54.rs:75-88) */
+    DRIFT_POTENTIAL_KEPLER = 1,       /* shared/src/potential.rs:217-239 */
+    DRIFT_POTENTIAL_PLUMMER = 2,      /* shared/src/potential.rs:241-262 */
+    DRIFT_POTENTIAL_MW = 3            /* bulge + Miyamoto-Nagai disk + NFW halo,
+                                         shared/src/potential.rs:87-215 */
+} drift_potential_kind;
+
+typedef enum drift_precision {
+    DRIFT_PRECISION_FP64 = 0,  /* everything in FP64 (default) */
+    DRIFT_PRECISION_MIXED = 1  /* stage forces in FP32; state update, error norm and
+                                  step control in FP64 (requires rtol >= 1e-7) */
+} drift_precision;
+
+/* Potential descriptor (G = 1 natural units).  Unused fields are ignored.
+ * MW: bulge rho = bulge_amp r^-alpha exp(-(r/rc)^2) (0 <= alpha < 3),
+ *     Miyamoto-Nagai (disk_amp, disk_a, disk_b), NFW (halo_amp, halo_a). */
+typedef struct drift_potential {
+    int32_t kind;      /* drift_potential_kind */
+    int32_t reserved;  /* must be 0 */
+    double amp;        /* Kepler / Plummer amplitude */
+    double b;          /* Plummer scale radius */
+    double bulge_amp, bulge_alpha, bulge_rc;
+    double disk_amp, disk_a, disk_b;
+    double halo_amp, halo_a;
+} drift_potential;
+
+/* Per-orbit statistics.  status holds the reference DOPR54 error code in its
+ * low byte (dopr54.rs:407-418; always 0 for DOP853) plus flags. */
+typedef struct drift_orbit_stats {
+    uint32_t accepted;    /* accepted steps */
+    uint32_t rejected;    /* rejected step attempts */
+    uint32_t dense_steps; /* DOP853: accepted steps that evaluated the dense stages */
+    int32_t status;
+} drift_orbit_stats;
+
+#define DRIFT_STATUS_CODE_MASK 0xff
+#define DRIFT_STATUS_NONFINITE 0x100   /* a non-finite state was written */
+#define DRIFT_STATUS_STEP_LIMIT 0x200  /* max_attempts reached; later samples are NaN */
+
+/* Launch tuning knobs for the device-resident entry points. */
+typedef struct drift_launch_options {
+    int32_t precision;  /* drift_precision */
+    int32_t block_size; /* threads per block; 0 = library default */
+    int32_t schedule;   /* 0 = one lane per orbit (plain grid),
+                           1 = persistent kernel, lanes refilled from a work queue */
+    uint32_t max_attempts; /* per-orbit attempted-step limit; 0 = default (2^22).
+                              The reference has none: a DOP853 orbit whose error
+                              estimate never drops below 1 retries forever at the
+                              step floor. */
+} drift_launch_options;
+
+typedef struct drift_session drift_session;
+
+/* ---- library / session -------------------------------------------------- */
+
+int drift_get_abi_version(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char* drift_last_error(void);
+
+/* Creates a session bound to HIP device `device` with its own stream
+ * (replaces the per-launch CudaContext::new(0), gpu/cuda_oxide.rs:16-18). */
+int drift_session_create(int device, drift_session** out);
+int drift_session_destroy(drift_session* session);
+
+/* ---- host-buffer launches (GpuKernel::GalpyDopr54 / GalpyDop853 through
+ *      backend::launch, src/dispatch/gpu/cuda_oxide.rs:9-86) ------------------ */
+
+/* DOPR54.  Copies state0 and times to the device, integrates every orbit,
+ * copies the time-major trajectories back into state_out.  stats (nullable)
+ * receives n records.  Returns DRIFT_ERR_INTEGRATION if any orbit's DOPR54
+ * error code is non-zero (outputs are still written), mirroring
+ * integrate_kepler's Err(code) (dopr54.rs:123). */
+int drift_galpy_dopr54(drift_session* session, const double* state0, const double* times,
+                       double* state_out, size_t n, size_t nt, double rtol, double atol,
+                       double dt_one_init, const drift_potential* potential,
+                       drift_orbit_stats* stats);
+
+/* DOP853, dense output at every times[i] (dop853.rs:747-763). */
+int drift_galpy_dop853(drift_session* session, const double* state0, const double* times,
+                       double* state_out, size_t n, size_t nt, double rtol, double atol,
+                       const drift_potential* potential, drift_orbit_stats* stats);
+
+/* ---- device-resident launches (inputs already in HBM) -------------------- */
+
+/* Same contracts with device pointers, enqueued on `stream` (a hipStream_t;
+ * NULL = the session's stream).  Asynchronous: no host synchronisation, no
+ * error-code scan; d_stats (nullable) receives per-orbit records on the
+ * device. */
+int drift_galpy_dopr54_device(drift_session* session, const double* d_state0,
+                              const double* d_times, double* d_state_out, size_t n, size_t nt,
+                              double rtol, double atol, double dt_one_init,
+                              const drift_potential* potential, drift_orbit_stats* d_stats,
+                              const drift_launch_options* options, void* stream);
+
+int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
+                              const double* d_times, double* d_state_out, size_t n, size_t nt,
+                              double rtol, double atol, const drift_potential* potential,
+                              drift_orbit_stats* d_stats, const drift_launch_options* options,
+                              void* stream);
+
+/* ---- force evaluation (test and diagnostics entry) ----------------------- */
+
+/* Accelerations of the device force formulation at n points.
+ * d_pos, d_acc: n x 3 doubles on the device. */
+int drift_force_device(drift_session* session, const drift_potential* potential,
+                       const double* d_pos, double* d_acc, size_t n, int32_t precision,
+                       void* stream);
+
+/* Host-only check of the Milky-Way spherical force table: worst relative error
+ * of the tabulated S(r^2) against its long-double evaluation over
+ * samples_per_segment points per table segment.  No device needed. */
+int drift_mw_table_check(const drift_potential* potential, int32_t samples_per_segment,
+                         double* max_rel_error);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DRIFT_HIP_H */

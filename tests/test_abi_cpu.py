@@ -1,0 +1,58 @@
+"""The C ABI library loads on a host without a GPU and exports every symbol
+include/drift_hip.h declares.  Only host-only entry points are called."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+
+import pytest
+
+from astrodrift_amd import _native
+
+
+def declared_functions() -> list[str]:
+    text = _native.HEADER_PATH.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"^\s*(?:const\s+)?[a-z_]+\**\s+\**(drift_[a-z0-9_]+)\s*\(", text, flags=re.M)
+
+
+def test_header_declares_the_expected_entry_points():
+    names = set(declared_functions())
+    assert {
+        "drift_get_abi_version", "drift_last_error", "drift_session_create",
+        "drift_session_destroy", "drift_galpy_dopr54", "drift_galpy_dop853",
+        "drift_galpy_dopr54_device", "drift_galpy_dop853_device", "drift_force_device",
+        "drift_mw_table_check",
+    } <= names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.load_library()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+        assert ctypes.cast(getattr(lib, name), ctypes.c_void_p).value
+
+
+def test_abi_version_and_struct_layouts():
+    lib = _native.load_library()
+    assert lib.drift_get_abi_version() == 1
+    assert ctypes.sizeof(_native.CPotential) == 8 + 10 * 8
+    assert ctypes.sizeof(_native.COrbitStats) == 16
+    assert ctypes.sizeof(_native.CLaunchOptions) == 16
+    assert _native.STATS_DTYPE.itemsize == 16
+
+
+def test_table_check_validates_its_arguments():
+    with pytest.raises(_native.DriftNativeError):
+        _native.mw_table_check(_native.PotentialSpec.kepler(1.0))
+    bad = _native.PotentialSpec.milky_way(bulge_alpha=3.5)
+    with pytest.raises(_native.DriftNativeError, match="alpha"):
+        _native.mw_table_check(bad)
+
+
+def test_null_session_is_an_error_not_a_crash():
+    lib = _native.load_library()
+    rc = lib.drift_galpy_dop853(None, None, None, None, 1, 2, 0.0, 0.0, None, None)
+    assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT
+    assert "session" in _native.last_error()

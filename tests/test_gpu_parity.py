@@ -1,0 +1,232 @@
+"""GPU parity: the gfx950 kernels through the C ABI against the CPU oracle.
+
+Bar (the reference's own): sampled and final phase-space agree with the
+oracle within 1e-8 relative (position and velocity norms separately), on the
+same inputs and tolerances; accepted/rejected step counts are reported.
+"""
+
+from __future__ import annotations
+
+import math
+import struct
+
+import numpy as np
+import pytest
+
+import astrodrift_amd as dft
+from astrodrift_amd import _native, workloads
+
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-8
+
+
+def rel_err(gpu: np.ndarray, cpu: np.ndarray) -> tuple[float, float]:
+    """Worst per-sample relative error of positions and of velocities."""
+    dp = np.linalg.norm(gpu[..., :3] - cpu[..., :3], axis=-1)
+    dv = np.linalg.norm(gpu[..., 3:] - cpu[..., 3:], axis=-1)
+    p = np.linalg.norm(cpu[..., :3], axis=-1)
+    v = np.linalg.norm(cpu[..., 3:], axis=-1)
+    return float(np.max(dp / np.maximum(p, 1e-300))), float(np.max(dv / np.maximum(v, 1e-300)))
+
+
+def bits(x: float) -> int:
+    return struct.unpack("<Q", struct.pack("<d", x))[0]
+
+
+KAT_TOL = struct.unpack("<d", struct.pack("<Q", 0xC03BA18A998FFFA1))[0]
+KAT_TAIL = [0xBFEAD9AC890CBF36, 0xBFE1689EF5F2F535, 0, 0x3FE1689EF5F2DA37, 0xBFEAD9AC890CA90A, 0]
+
+
+def test_dopr54_canonical_kepler_kat(gpu):
+    """tests/galpy_integrator_tests.rs:35-42, 70-102 through the C ABI."""
+    y0 = np.array([[1.0, 0.0, 0.0, 0.0, 1.0, 0.0]])
+    t = np.linspace(0.0, 10.0, 1001)
+    out, stats, code = _native.launch_host(_native.METHOD_DOPR54, y0, t, KAT_TOL, KAT_TOL)
+    assert code == 0
+    expected = np.array([struct.unpack("<d", struct.pack("<Q", b))[0] for b in KAT_TAIL])
+    tail = out[-1, 0]
+    print("GPU tail", [hex(bits(v)) for v in tail], "max |diff|", np.max(np.abs(tail - expected)))
+    np.testing.assert_allclose(tail, expected, rtol=0, atol=1e-12)
+    cpu, st, err = oracle_lib.orbit(0, y0[0], t, KAT_TOL, KAT_TOL)
+    assert err == 0
+    np.testing.assert_allclose(out[:, 0], cpu, rtol=0, atol=1e-12)
+    assert stats["accepted"][0] == st.accepted and stats["rejected"][0] == st.rejected
+
+
+def _compare(method: int, w: workloads.Workload, idx=None, label=""):
+    y0 = w.state0 if idx is None else w.state0[idx]
+    rl, al = math.log(w.rtol), math.log(w.atol)
+    out, stats, code = _native.launch_host(method, y0, w.times, rl, al, w.potential)
+    cpu, cst = oracle_lib.batch(method, y0, w.times, rl, al, oracle_lib_spec(w.potential),
+                                nthreads=8)
+    ep, ev = rel_err(out, cpu)
+    acc_g, acc_c = int(stats["accepted"].sum()), int(cst["accepted"].sum())
+    rej_g, rej_c = int(stats["rejected"].sum()), int(cst["rejected"].sum())
+    same = float(np.mean(stats["accepted"] == cst["accepted"]))
+    print(f"{label}: n={y0.shape[0]} pos_rel={ep:.3e} vel_rel={ev:.3e} accepted gpu/cpu="
+          f"{acc_g}/{acc_c} rejected gpu/cpu={rej_g}/{rej_c} same-count orbits={same:.4f}")
+    assert np.array_equal(out[0], y0)
+    assert ep < REL_TOL and ev < REL_TOL
+    assert abs(acc_g - acc_c) <= 1e-3 * acc_c
+    return out, cpu, stats, cst
+
+
+def oracle_lib_spec(spec):
+    return spec
+
+
+def test_kepler_disk_dopr54_parity(gpu):
+    """1,024 Kepler orbits, DOPR54, rtol = atol = 1e-10, ts = (0, 20 pi, 101)."""
+    w = workloads.kepler_disk()
+    _compare(_native.METHOD_DOPR54, w, label="kepler_disk DOPR54")
+
+
+def test_mw_disk_dop853_subset_parity(gpu):
+    """A strided 256-orbit subset of the 1M-orbit Milky-Way DOP853 workload."""
+    w = workloads.mw_disk()
+    idx = np.arange(0, w.n, w.n // 256)
+    out, cpu, _, _ = _compare(_native.METHOD_DOP853, w, idx, label="mw_disk DOP853 subset")
+    # energy and Lz drift of GPU and oracle trajectories match
+    spec = w.potential
+
+    def energy(tr):
+        e = np.empty(tr.shape[:2])
+        for s in range(tr.shape[0]):
+            for k in range(tr.shape[1]):
+                x = tr[s, k]
+                e[s, k] = 0.5 * (x[3] ** 2 + x[4] ** 2 + x[5] ** 2) + oracle_lib.potential_value(
+                    spec, x[0], x[1], x[2])
+        return e
+
+    sel = [0, 500, 1000]
+    eg, ec = energy(out[sel]), energy(cpu[sel])
+    dEg = np.abs((eg[-1] - eg[0]) / eg[0])
+    dEc = np.abs((ec[-1] - ec[0]) / ec[0])
+    lz = lambda tr: tr[..., 0] * tr[..., 4] - tr[..., 1] * tr[..., 3]
+    dLg = np.abs(lz(out[-1]) - lz(out[0])) / np.abs(lz(out[0]))
+    dLc = np.abs(lz(cpu[-1]) - lz(cpu[0])) / np.abs(lz(cpu[0]))
+    print(f"|dE/E| gpu max {dEg.max():.3e} cpu max {dEc.max():.3e}; "
+          f"|dLz/Lz| gpu max {dLg.max():.3e} cpu max {dLc.max():.3e}")
+    assert dEg.max() < 1e-9 and dLg.max() < 1e-9
+    assert dEg.max() < 10 * dEc.max() + 1e-12
+
+
+@pytest.mark.parametrize("method", [_native.METHOD_DOPR54, _native.METHOD_DOP853])
+@pytest.mark.parametrize("pot", ["galpy", "kepler", "plummer", "mw"])
+def test_small_parity_matrix(gpu, method, pot):
+    """Every (method, potential) pair on 96 orbits (a ragged count)."""
+    specs = {
+        "galpy": None,
+        "kepler": _native.PotentialSpec.kepler(1.3),
+        "plummer": _native.PotentialSpec.plummer(1.0, 0.1),
+        "mw": _native.PotentialSpec.milky_way(),
+    }
+    w = workloads.mw_disk(n=96, seed=7, t_end=10.0, steps=201, rtol=1e-10, atol=1e-10)
+    w.potential = specs[pot]
+    _compare(method, w, label=f"{pot} m{method}")
+
+
+@pytest.mark.parametrize("method", [_native.METHOD_DOPR54, _native.METHOD_DOP853])
+def test_reference_smoke_orbit_closure(gpu, method):
+    """tests/test_python_smoke.py:159-225: 20 circular orbits, one period."""
+    gmc = np.tile([1.0, 0.0, 0.0, 0.0, 1.0, 0.0], (20, 1))
+    iso = np.tile([-1.0, 0.0, 0.0, 0.0, -1.0, 0.0], (20, 1))
+    kp = dft.Potential.kepler(1.0)
+    gal = dft.background(kp)
+    g = dft.particles(kp, gmc)
+    i = dft.test_particles(iso)
+    sim = dft.Config(engine=dft.Engine.GPU,
+                     method=dft.Method.DOPR54 if method == 0 else dft.Method.DOP853)
+    sim.add(g, gal)
+    sim.add(i, g, gal)
+    gmc_tr, bg, iso_tr = sim.run()
+    assert bg is None
+    t = dft.OutputGrid().times()
+    tol = math.log(1e-12)
+    for tr, init in ((iso_tr, iso), (gmc_tr, gmc)):
+        assert tr.shape == (100, 20, 6) and tr.dtype == np.float64
+        np.testing.assert_array_equal(tr[0], init)
+        assert not np.array_equal(tr[1], init)
+        np.testing.assert_allclose(tr[-1], init, rtol=0.0, atol=2.0e-10)
+        cpu, _, _ = oracle_lib.orbit(method, init[0], t, tol, tol)
+        for k in range(20):
+            np.testing.assert_allclose(tr[:, k], cpu, rtol=1e-12, atol=1e-12)
+
+
+def test_plunge_raises_integration_error(gpu):
+    """tests/test_python_errors.py:8-40 on the GPU engine."""
+    sim = dft.Config(engine=dft.Engine.GPU, method=dft.Method.DOPR54, ts=(0.0, 5.0, 64),
+                     tolerance=(1.0e-13, 1.0e-13))
+    sim.add(dft.test_particles(np.array([[1.0, 0.0, 0.0, -3.0, 0.0, 0.0]])),
+            dft.background(dft.Potential.kepler(1.0)))
+    with pytest.raises(dft.IntegrationError, match="error code") as raised:
+        sim.run()
+    assert isinstance(raised.value, dft.DriftError)
+    # the oracle reports the same error code
+    _, _, err = oracle_lib.orbit(0, [1.0, 0.0, 0.0, -3.0, 0.0, 0.0], workloads.grid_times(0, 5, 64),
+                                 math.log(1e-13), math.log(1e-13))
+    assert err != 0
+
+
+@pytest.mark.parametrize("method", [_native.METHOD_DOPR54, _native.METHOD_DOP853])
+def test_edge_shapes(gpu, method):
+    """n = 1 with nt = 2; backwards time; nt beyond the reference's 1024."""
+    tol = math.log(1e-10)
+    y0 = np.array([[0.9, 0.1, 0.05, -0.1, 1.0, 0.02]])
+    for times in (np.array([0.0, 1.5]), workloads.grid_times(3.0, -2.0, 17),
+                  workloads.grid_times(0.0, 30.0, 3001)):
+        out, stats, code = _native.launch_host(method, y0, times, tol, tol)
+        cpu, st, err = oracle_lib.orbit(method, y0[0], times, tol, tol)
+        assert code == 0 and err == 0
+        ep, ev = rel_err(out[:, 0], cpu)
+        assert ep < REL_TOL and ev < REL_TOL, (times.size, ep, ev)
+
+
+def test_step_limit_is_flagged(gpu):
+    """A host-forced tiny attempt limit ends the orbit and flags it."""
+    torch = pytest.importorskip("torch")
+    w = workloads.kepler_disk(n=64)
+    dev = torch.device("cuda:0")
+    s0 = torch.from_numpy(w.state0).to(dev)
+    t = torch.from_numpy(w.times).to(dev)
+    out = torch.zeros((w.nt, w.n, 6), dtype=torch.float64, device=dev)
+    stats = torch.zeros((w.n, 4), dtype=torch.int32, device=dev)
+    tol = math.log(w.rtol)
+    _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), t.data_ptr(), out.data_ptr(),
+                          w.n, w.nt, tol, tol, None, stats.data_ptr(), max_attempts=5)
+    torch.cuda.synchronize()
+    st = stats.cpu().numpy()
+    assert np.all(st[:, 3] & _native.STATUS_STEP_LIMIT)
+    assert np.all(st[:, 0] + st[:, 1] == 5)
+    assert torch.isnan(out[-1]).all()
+
+
+@pytest.mark.parametrize("precision", [_native.PRECISION_FP64, _native.PRECISION_MIXED])
+@pytest.mark.parametrize("pot", ["galpy", "kepler", "plummer", "mw"])
+def test_force_parity(gpu, precision, pot):
+    """Device force formulations against the oracle's restated formulas."""
+    torch = pytest.importorskip("torch")
+    specs = {
+        "galpy": None,
+        "kepler": _native.PotentialSpec.kepler(2.0),
+        "plummer": _native.PotentialSpec.plummer(1.0, 0.1),
+        "mw": _native.PotentialSpec.milky_way(),
+    }
+    spec = specs[pot]
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-3, 3, (4096, 3))
+    pos[:16] *= 1e-4  # near the centre (MW slow path below the table)
+    pos[16:32] *= 40.0  # far out (MW slow path above the table)
+    dp = torch.from_numpy(pos).cuda()
+    da = torch.zeros_like(dp)
+    _native.force_device(spec, dp.data_ptr(), da.data_ptr(), pos.shape[0], precision)
+    torch.cuda.synchronize()
+    acc = da.cpu().numpy()
+    ref = np.array([oracle_lib.force(spec, *p) for p in pos])
+    rel = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    bound = 1e-13 if precision == _native.PRECISION_FP64 else 2e-6
+    print(f"{pot} precision={precision}: max rel force error {rel.max():.3e}")
+    assert rel.max() < bound

@@ -1,0 +1,167 @@
+"""Pins the oracle's force formulas (and the MW model constants).
+
+* Plummer: the reference's property tests
+  (shared/tests/plummer_potential_tests.rs:52-222), restated.
+* Every potential: force = -grad(Phi) by central finite differences.
+* Milky-Way model: the vc^2(R=1) fractions 0.05 / 0.6 / 0.35 that define
+  its amplitudes (shared/src/potential.rs:104, :110 give the disk and halo
+  amplitudes), Lz-conserving axisymmetry, and the bulge enclosed mass against
+  scipy.special.gammainc.
+* The table the kernels use for the MW spherical part, checked on the host.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+from scipy.special import gamma as gamma_fn
+from scipy.special import gammainc
+
+from astrodrift_amd import _native
+
+import oracle_lib
+
+MW = _native.PotentialSpec.milky_way()
+
+
+def plummer(amp, b):
+    return _native.PotentialSpec.plummer(amp, b)
+
+
+def f(spec, x, y, z):
+    return oracle_lib.force(spec, x, y, z)
+
+
+def test_plummer_zero_force_at_origin_and_finite_nearby():
+    p = plummer(1.0, 0.6)
+    np.testing.assert_allclose(f(p, 0.0, 0.0, 0.0), 0.0, atol=1e-15)
+    fn = f(p, 1e-6, -1e-6, 2e-6)
+    assert np.all(np.isfinite(fn)) and np.linalg.norm(fn) < 1.0
+
+
+def test_plummer_odd_symmetry():
+    p = plummer(1.0, 0.5)
+    np.testing.assert_allclose(f(p, 0.7, -0.3, 1.1), -f(p, -0.7, 0.3, -1.1), atol=1e-12)
+
+
+def test_plummer_central_direction():
+    p = plummer(1.0, 0.4)
+    r = np.array([0.8, 1.2, -0.5])
+    fr = f(p, *r)
+    cos = fr @ r / (np.linalg.norm(fr) * np.linalg.norm(r))
+    assert abs(cos + 1.0) < 1e-12
+
+
+@pytest.mark.parametrize(("r1", "r2"), [((1, 0, 0), (0, 1, 0)), ((1, 0, 0), (0, 0, 1)),
+                                        ((1, 1, 0), (1, 0, 1)), ((2, -1, 0.5), (-1, 2, 0.5))])
+def test_plummer_isotropy(r1, r2):
+    p = plummer(1.0, 0.5)
+    r1, r2 = np.array(r1, float), np.array(r2, float)
+    r2 = r2 * np.linalg.norm(r1) / np.linalg.norm(r2)
+    m1, m2 = np.linalg.norm(f(p, *r1)), np.linalg.norm(f(p, *r2))
+    assert abs(m1 - m2) <= 1e-12 * m1
+
+
+def test_plummer_magnitude_peaks_at_b_over_sqrt2():
+    b = 0.5
+    p = plummer(1.0, b)
+    d = np.array([0.6, -0.2, 0.77])
+    d /= np.linalg.norm(d)
+    rp = b / math.sqrt(2.0)
+    below = [np.linalg.norm(f(p, *(d * r))) for r in (0.2 * rp, 0.5 * rp, 0.9 * rp)]
+    above = [np.linalg.norm(f(p, *(d * r))) for r in (1.1 * rp, 2.0 * rp, 4.0 * rp)]
+    assert below == sorted(below) and above == sorted(above, reverse=True)
+
+
+def test_plummer_amp_linearity_and_far_field():
+    r = (0.3, -1.7, 2.2)
+    np.testing.assert_allclose(f(plummer(3.0, 0.5), *r), 3.0 * f(plummer(1.0, 0.5), *r),
+                               atol=1e-12)
+    p = plummer(2.0, 0.4)
+    r1 = np.array([50 / 3, 100 / 3, 0.0])
+    ratio = np.linalg.norm(f(p, *(2.5 * r1))) / np.linalg.norm(f(p, *r1))
+    assert abs(ratio / (1 / 2.5**2) - 1) < 3e-3
+
+
+def test_plummer_randomized_centrality_and_rotation():
+    p = plummer(1.23, 0.37)
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        x, y, z = rng.uniform(-3, 3, 3)
+        fr = f(p, x, y, z)
+        cos = fr @ np.array([x, y, z]) / (np.linalg.norm(fr) * math.sqrt(x * x + y * y + z * z))
+        assert cos < -0.999999
+        th = rng.uniform(0, 2 * math.pi)
+        g = f(p, x * math.cos(th) - y * math.sin(th), x * math.sin(th) + y * math.cos(th), z)
+        assert abs(np.linalg.norm(g) - np.linalg.norm(fr)) <= 1e-12 * np.linalg.norm(fr)
+
+
+@pytest.mark.parametrize("spec", [None, _native.PotentialSpec.kepler(1.7), plummer(1.0, 0.1), MW],
+                         ids=["galpy", "kepler", "plummer", "mw"])
+def test_force_is_minus_grad_phi(spec):
+    rng = np.random.default_rng(9)
+    h = 1e-5
+    for _ in range(40):
+        p = rng.uniform(-2.0, 2.0, 3)
+        if np.linalg.norm(p) < 0.05:
+            continue
+        g = np.zeros(3)
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = h
+            g[k] = -(oracle_lib.potential_value(spec, *(p + e))
+                     - oracle_lib.potential_value(spec, *(p - e))) / (2 * h)
+        fo = f(spec, *p)
+        assert np.linalg.norm(fo - g) < 1e-8 * max(1.0, np.linalg.norm(fo)), (p, fo, g)
+
+
+def test_mw_circular_speed_fractions_at_r0():
+    """vc^2(1) of bulge, disk, halo = 0.05, 0.6, 0.35 and vc(1) = 1."""
+    # one component at a time; the others get zero amplitude (finite scales)
+    dummy = dict(bulge_alpha=1.8, bulge_rc=1.0, disk_a=1.0, disk_b=1.0, halo_a=1.0)
+    parts = {
+        "bulge": _native.PotentialSpec(3, **{**dummy, "bulge_amp": MW.bulge_amp,
+                                             "bulge_alpha": MW.bulge_alpha,
+                                             "bulge_rc": MW.bulge_rc}),
+        "disk": _native.PotentialSpec(3, **{**dummy, "disk_amp": MW.disk_amp,
+                                            "disk_a": MW.disk_a, "disk_b": MW.disk_b}),
+        "halo": _native.PotentialSpec(3, **{**dummy, "halo_amp": MW.halo_amp,
+                                            "halo_a": MW.halo_a}),
+    }
+    vc2 = {k: -f(s, 1.0, 0.0, 0.0)[0] for k, s in parts.items()}
+    assert abs(vc2["bulge"] - 0.05) < 1e-13
+    assert abs(vc2["disk"] - 0.6) < 1e-13
+    assert abs(vc2["halo"] - 0.35) < 1e-13
+    assert abs(-f(MW, 1.0, 0.0, 0.0)[0] - 1.0) < 1e-13
+    # the reference's BovyPotential amplitudes (shared/src/potential.rs:104, 110)
+    assert abs(MW.disk_amp - 0.7574802019) < 1e-10
+    assert abs(MW.halo_amp - 4.852230533528) < 1e-11
+
+
+def test_mw_axisymmetric_torque_free():
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        x, y, z = rng.uniform(-2, 2, 3)
+        ax, ay, _ = f(MW, x, y, z)
+        assert abs(x * ay - y * ax) <= 1e-14 * math.hypot(x, y) * math.hypot(ax, ay) + 1e-300
+
+
+@pytest.mark.parametrize("a", [0.6, 0.1, 1.3])
+def test_lower_gamma_against_scipy(a):
+    lib = oracle_lib.load()
+    for x in np.concatenate([np.geomspace(1e-8, 200.0, 60), [a + 1.0]]):
+        got = lib.oracle_lower_gamma(a, x)
+        want = gammainc(a, x) * gamma_fn(a)
+        assert abs(got - want) <= 2e-14 * want, (x, got, want)
+        up = lib.oracle_upper_gamma(a, x)
+        assert abs(got + up - gamma_fn(a)) <= 1e-14 * gamma_fn(a)
+
+
+def test_mw_table_check_on_host():
+    """The kernels' tabulated bulge+halo force against its long-double form."""
+    assert _native.mw_table_check(MW, 400) < 1e-15
+    # a different parameter set builds its own table
+    other = _native.PotentialSpec.milky_way(bulge_alpha=1.2, bulge_rc=0.5, halo_a=0.8)
+    assert _native.mw_table_check(other, 200) < 1e-15
