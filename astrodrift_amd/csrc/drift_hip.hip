@@ -14,6 +14,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -54,6 +55,8 @@ struct PlainArgs {
     double bound2[6];     // DOPR54 only
     PotParams pot;
     const void* table;    // MW: TABLE_DOUBLES coefficients (double or float)
+    unsigned long long* queue;  // persistent schedule: next unclaimed orbit slot
+    const uint32_t* order;      // persistent schedule: slot -> orbit (NULL = identity)
 };
 
 template <int KIND, class FT>
@@ -164,6 +167,110 @@ __global__ void __launch_bounds__(kDefaultBlock) dopr54_plain_kernel(PlainArgs a
     }
 }
 
+// Persistent schedule: a grid sized to the machine; each lane integrates
+// orbits one after another, claiming the next slot from a global queue when
+// its orbit finishes.  Requests are aggregated per wavefront with a ballot so
+// one atomic serves every lane that needs work, and lanes stay busy until the
+// queue drains instead of idling behind the wave's slowest orbit.
+__device__ __forceinline__ bool claim_orbit(bool need, unsigned long long* queue, int64_t n,
+                                            const uint32_t* order, int64_t& orbit) {
+    const unsigned long long want = __ballot(need);
+    if (want == 0) return true;
+    const int lane = __lane_id();
+    const int leader = __ffsll(want) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(queue, (unsigned long long)__popcll(want));
+    base = __shfl(base, leader);
+    if (!need) return true;
+    const unsigned long long below = lane == 0 ? 0ull : (want & (~0ull >> (64 - lane)));
+    const unsigned long long slot = base + (unsigned long long)__popcll(below);
+    if (slot >= (unsigned long long)n) return false;  // queue drained: lane retires
+    orbit = order ? (int64_t)order[slot] : (int64_t)slot;
+    return true;
+}
+
+template <int KIND, class FT>
+__global__ void __launch_bounds__(kDefaultBlock) dop853_persistent_kernel(PlainArgs a) {
+    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    const FT* table = stage_table<KIND, FT>(a.table, lds);
+    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    const double t0 = a.times[0];
+    const Dop853Job J = make_dop853_job(a.times, a.nt, a.rtol, a.atol, a.max_attempts, t0,
+                                        a.times[a.nt - 1]);
+    OutSink sink{a.out, a.n, 0};
+    Dop853Lane L;
+    bool need = true;
+    for (;;) {
+        int64_t orbit = -1;
+        if (!claim_orbit(need, a.queue, a.n, a.order, orbit)) break;
+        if (need) {
+            sink.idx = orbit;
+            double y0[DIM];
+            load_state(a.state0, orbit, y0);
+            dop853_init(L, y0, t0, J, f, sink);
+            need = false;
+        }
+        if (dop853_attempt(L, J, f, sink)) {
+            if (a.stats) {
+                drift_orbit_stats st;
+                st.accepted = L.accepted;
+                st.rejected = L.rejected;
+                st.dense_steps = L.dense;
+                st.status = L.status;
+                a.stats[sink.idx] = st;
+            }
+            need = true;
+        }
+    }
+}
+
+template <int KIND, class FT>
+__global__ void __launch_bounds__(kDefaultBlock) dopr54_persistent_kernel(PlainArgs a) {
+    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    const FT* table = stage_table<KIND, FT>(a.table, lds);
+    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    Dopr54Job J;
+    J.times = a.times;
+    J.nt = a.nt;
+    J.max_attempts = a.max_attempts;
+    J.dt = a.times[1] - a.times[0];
+    J.rtol = a.rtol;
+    J.atol = a.atol;
+    J.dt_one_init = a.dt_one_init;
+#pragma unroll
+    for (int k = 0; k < 6; k++) J.bound2[k] = a.bound2[k];
+    OutSink sink{a.out, a.n, 0};
+    Dopr54Lane L;
+    bool need = true;
+    for (;;) {
+        int64_t orbit = -1;
+        if (!claim_orbit(need, a.queue, a.n, a.order, orbit)) break;
+        bool done = false;
+        if (need) {
+            sink.idx = orbit;
+            double y0[DIM];
+            load_state(a.state0, orbit, y0);
+            dopr54_init(L, y0, a.times[0], J, f, sink);
+            need = false;
+            done = L.ii >= J.nt - 1;
+        }
+        if (!done) done = dopr54_attempt(L, J, f, sink);
+        if (done) {
+            if (a.stats) {
+                drift_orbit_stats st;
+                st.accepted = L.accepted;
+                st.rejected = L.rejected;
+                st.dense_steps = 0;
+                st.status = L.err | L.status;
+                a.stats[sink.idx] = st;
+            }
+            need = true;
+        }
+    }
+}
+
+__global__ void reset_queue_kernel(unsigned long long* queue) { *queue = 0ull; }
+
 struct ForceArgs {
     const double* pos;
     double* acc;
@@ -215,9 +322,15 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 struct drift_session {
     int device = 0;
+    int cu_count = 0;
     hipStream_t stream = nullptr;
     std::vector<drift::TableEntry> tables;
     drift::DeviceBuffer state0, times, out, stats;
+    // ring of work-queue counters for persistent launches (one per launch in
+    // flight; launches further apart than the ring reuse a counter only after
+    // stream order has retired its previous user)
+    unsigned long long* queues = nullptr;
+    unsigned queue_next = 0;
 };
 
 namespace drift {
@@ -335,23 +448,29 @@ int resolve_potential(drift_session* s, const drift_potential* potential, PotPar
 
 using PlainKernel = void (*)(PlainArgs);
 
-template <int KIND>
-PlainKernel pick_plain(int method, int precision) {
+template <int KIND, class FT>
+PlainKernel pick_schedule(int method, int schedule) {
     if (method == DRIFT_METHOD_DOP853)
-        return precision == DRIFT_PRECISION_MIXED ? dop853_plain_kernel<KIND, float>
-                                                  : dop853_plain_kernel<KIND, double>;
-    return precision == DRIFT_PRECISION_MIXED ? dopr54_plain_kernel<KIND, float>
-                                              : dopr54_plain_kernel<KIND, double>;
+        return schedule ? dop853_persistent_kernel<KIND, FT> : dop853_plain_kernel<KIND, FT>;
+    return schedule ? dopr54_persistent_kernel<KIND, FT> : dopr54_plain_kernel<KIND, FT>;
 }
 
-PlainKernel plain_kernel_for(int method, int kind, int precision) {
+template <int KIND>
+PlainKernel pick_plain(int method, int precision, int schedule) {
+    return precision == DRIFT_PRECISION_MIXED ? pick_schedule<KIND, float>(method, schedule)
+                                              : pick_schedule<KIND, double>(method, schedule);
+}
+
+PlainKernel kernel_for(int method, int kind, int precision, int schedule) {
     switch (kind) {
-        case POT_GALPY_KEPLER: return pick_plain<POT_GALPY_KEPLER>(method, precision);
-        case POT_KEPLER: return pick_plain<POT_KEPLER>(method, precision);
-        case POT_PLUMMER: return pick_plain<POT_PLUMMER>(method, precision);
-        default: return pick_plain<POT_MW>(method, precision);
+        case POT_GALPY_KEPLER: return pick_plain<POT_GALPY_KEPLER>(method, precision, schedule);
+        case POT_KEPLER: return pick_plain<POT_KEPLER>(method, precision, schedule);
+        case POT_PLUMMER: return pick_plain<POT_PLUMMER>(method, precision, schedule);
+        default: return pick_plain<POT_MW>(method, precision, schedule);
     }
 }
+
+constexpr unsigned kQueueRing = 64;
 
 constexpr uint32_t kDefaultMaxAttempts = 1u << 22;
 constexpr double kMixedMinRtol = 1e-7;
@@ -378,8 +497,8 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     int block = opt.block_size ? opt.block_size : kDefaultBlock;
     if (block < 64 || block > kDefaultBlock || block % 64 != 0)
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
-    if (opt.schedule != 0)
-        return fail(DRIFT_ERR_UNSUPPORTED, "schedule must be 0 (plain grid)");
+    if (opt.schedule != 0 && opt.schedule != 1)
+        return fail(DRIFT_ERR_UNSUPPORTED, "schedule must be 0 (plain grid) or 1 (persistent)");
 
     PlainArgs a{};
     a.state0 = d_state0;
@@ -404,10 +523,21 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     if (rc != DRIFT_OK) return rc;
     a.table = opt.precision == DRIFT_PRECISION_MIXED ? t32 : t64;
 
-    PlainKernel k = plain_kernel_for(method, a.pot.kind, opt.precision);
-    const int64_t blocks = ((int64_t)n + block - 1) / block;
+    PlainKernel k = kernel_for(method, a.pot.kind, opt.precision, opt.schedule);
+    hipStream_t st = stream ? stream : s->stream;
+    int64_t blocks = ((int64_t)n + block - 1) / block;
+    if (opt.schedule == 1) {
+        if (!s->queues) DRIFT_HIP_TRY(hipMalloc(&s->queues, kQueueRing * sizeof(unsigned long long)));
+        a.queue = s->queues + (s->queue_next++ % kQueueRing);
+        int per_cu = 0;
+        DRIFT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0));
+        const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(s->cu_count, 1);
+        blocks = std::min(blocks, resident);
+        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, st, a.queue);
+        DRIFT_HIP_TRY(hipGetLastError());
+    }
     if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");
-    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, stream ? stream : s->stream, a);
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, st, a);
     DRIFT_HIP_TRY(hipGetLastError());
     return DRIFT_OK;
 }
@@ -481,6 +611,9 @@ int drift_session_create(int device, drift_session** out) {
     DRIFT_HIP_TRY(hipSetDevice(device));
     drift_session* s = new drift_session();
     s->device = device;
+    if (hipDeviceGetAttribute(&s->cu_count, hipDeviceAttributeMultiprocessorCount, device) !=
+        hipSuccess)
+        s->cu_count = 256;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete s;
@@ -500,6 +633,7 @@ int drift_session_destroy(drift_session* s) {
     }
     for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats})
         if (b->ptr) (void)hipFree(b->ptr);
+    if (s->queues) (void)hipFree(s->queues);
     (void)hipStreamDestroy(s->stream);
     delete s;
     return DRIFT_OK;

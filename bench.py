@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Throughput of the batched DOP853 integrator on one MI355X.
+
+Workload (one "step" = one integration of the whole batch, inputs resident in
+HBM): 1,048,576 orbits in the Milky-Way model (bulge + Miyamoto-Nagai disk +
+NFW halo), exponential-disk initial conditions, DOP853 with
+rtol = atol = 1e-12, 1,001 dense outputs over t = 0..50, FP64; the
+(1001, 1048576, 6) time-major output (50.4 GB) is written to HBM every step.
+
+Prints one JSON line: accepted particle-steps/s of the chosen schedule, the
+plain (one lane per orbit) figure beside it, the roofline of the integrator
+kernel (algorithmic FP64 FLOP/s from the per-orbit step statistics, over the
+kernel's HIP-event duration) and the CPU oracle timed on a strided sample of
+the same orbits.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+METRIC = "accepted particle-steps/s, 1M MW orbits DOP853, plain vs chosen; p50/p99 latency"
+UNIT = "accepted particle-steps/s"
+FP64_PEAK_TFLOPS = 78.6  # MI355X vector / matrix FP64
+
+
+def parse_args():
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=1 << 20, help="orbits (default 1,048,576)")
+    p.add_argument("--schedules", default="plain",
+                   help="comma list of schedules to time: plain,persistent")
+    p.add_argument("--cpu-stride", type=int, default=512,
+                   help="CPU baseline integrates every k-th orbit")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(w, stride: int):
+    """The oracle (the reference CPU algorithm restated) on orbits 0, stride,
+    2*stride, ...: one thread (as the reference engine), then all host cores."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import numpy as np
+
+    import oracle_lib
+
+    tol_r, tol_a = math.log(w.rtol), math.log(w.atol)
+    idx = np.arange(0, w.n, stride)
+    y0 = np.ascontiguousarray(w.state0[idx])
+    t0 = time.perf_counter()
+    _, st1 = oracle_lib.batch(1, y0, w.times, tol_r, tol_a, w.potential, nthreads=1,
+                              want_out=False)
+    one = time.perf_counter() - t0
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    t0 = time.perf_counter()
+    _, stn = oracle_lib.batch(1, y0, w.times, tol_r, tol_a, w.potential, nthreads=cores,
+                              want_out=False)
+    many = time.perf_counter() - t0
+    acc = int(st1["accepted"].sum())
+    return {
+        "value": acc / one,
+        "unit": UNIT,
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{len(idx)} orbits (every {stride}th of the {w.n}), DOP853 rtol=atol=1e-12, "
+                  f"t=0..50, {acc} accepted steps, {one:.1f} s on 1 thread",
+        "all_cores": {"value": int(stn["accepted"].sum()) / many, "cores": cores,
+                      "seconds": round(many, 2)},
+    }
+
+
+def main() -> int:
+    args = parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
+              "the orbits are independent and a multi-GPU run would shard them by range with no "
+              "communication, which this build does not launch.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from astrodrift_amd import _native, flops, workloads
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    w = workloads.mw_disk(n=args.n)
+    s0 = torch.from_numpy(w.state0).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    out = torch.empty((w.nt, w.n, 6), dtype=torch.float64, device=dev)
+    stats = torch.zeros((w.n, 4), dtype=torch.int32, device=dev)
+    tol_r, tol_a = math.log(w.rtol), math.log(w.atol)
+    stream = torch.cuda.current_stream()
+
+    sched_ids = {"plain": _native.SCHEDULE_PLAIN, "persistent": _native.SCHEDULE_PERSISTENT}
+    results = {}
+    for name in [s.strip() for s in args.schedules.split(",") if s.strip()]:
+        schedule = sched_ids[name]
+
+        def launch():
+            _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(),
+                                  out.data_ptr(), w.n, w.nt, tol_r, tol_a, w.potential,
+                                  stats.data_ptr(), schedule=schedule,
+                                  stream=stream.cuda_stream)
+
+        for _ in range(args.warmup):
+            launch()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        wall0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            launch()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall0
+        kernel_s = ev0.elapsed_time(ev1) / 1e3 / args.steps
+        st = stats.cpu().numpy()
+        accepted = int(st[:, 0].astype(np.int64).sum())
+        flop = flops.dop853_flops(st, w.nt, w.potential)
+        results[name] = {
+            "ms_per_step": wall / args.steps * 1e3,
+            "kernel_ms": kernel_s * 1e3,
+            "accepted": accepted,
+            "rejected": int(st[:, 1].astype(np.int64).sum()),
+            "dense_steps": int(st[:, 2].astype(np.int64).sum()),
+            "bad_status": int(np.count_nonzero(st[:, 3])),
+            "value": accepted / (wall / args.steps),
+            "tflops": flop / kernel_s / 1e12,
+            "flop_per_launch": flop,
+        }
+        print(f"[bench] {name}: {results[name]}", file=sys.stderr, flush=True)
+
+    chosen = max(results, key=lambda k: results[k]["value"])
+    r = results[chosen]
+    line = {
+        "metric": METRIC,
+        "value": r["value"],
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"mw_disk_dop853: {w.n} Milky-Way orbits (bulge+MN disk+NFW halo), "
+                        "exponential-disk ICs seed 0x100000, DOP853 rtol=atol=1e-12, "
+                        f"{w.nt} outputs over t=0..50, FP64, (nt, n, 6) output in HBM",
+            "schedule": chosen,
+            "plain_value": results.get("plain", {}).get("value"),
+            "schedules": results,
+        },
+        "roofline": {
+            "bound": "fp64-valu",
+            "achieved": r["tflops"],
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": r["tflops"] / FP64_PEAK_TFLOPS,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(w, args.cpu_stride)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
