@@ -216,6 +216,23 @@ class PotentialSpec:
         )
 
 
+class CJob(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int32),
+        ("potential", ctypes.c_int32),
+        ("n", ctypes.c_size_t),
+        ("nt", ctypes.c_size_t),
+        ("rtol", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("dt_one_init", ctypes.c_double),
+        ("span_hint", ctypes.c_double),
+        ("d_state0", ctypes.c_void_p),
+        ("d_times", ctypes.c_void_p),
+        ("d_state_out", ctypes.c_void_p),
+        ("d_stats", ctypes.c_void_p),
+    ]
+
+
 # ---------------------------------------------------------------------------
 # Library loading
 # ---------------------------------------------------------------------------
@@ -248,6 +265,10 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.drift_galpy_dop853_device.restype = c.c_int
     lib.drift_galpy_dop853_device.argtypes = host + [
         c.POINTER(CPotential), c.c_void_p, c.POINTER(CLaunchOptions), c.c_void_p]
+    lib.drift_ragged_device.restype = c.c_int
+    lib.drift_ragged_device.argtypes = [c.c_void_p, c.POINTER(CJob), c.c_size_t,
+                                        c.POINTER(CPotential), c.c_size_t,
+                                        c.POINTER(CLaunchOptions), c.c_void_p]
     lib.drift_force_device.restype = c.c_int
     lib.drift_force_device.argtypes = [c.c_void_p, c.POINTER(CPotential), c.c_void_p, c.c_void_p,
                                        c.c_size_t, c.c_int32, c.c_void_p]
@@ -405,6 +426,43 @@ def launch_device(
         code = s.lib.drift_galpy_dop853_device(*args, pot_ref, ctypes.c_void_p(d_stats),
                                                ctypes.byref(opts), ctypes.c_void_p(stream))
     check(code)
+
+
+@dataclass
+class DeviceJob:
+    """One job of a ragged batch, on device pointers."""
+
+    method: int
+    potential: int  # index into the potentials list
+    n: int
+    nt: int
+    rtol_ln: float
+    atol_ln: float
+    d_state0: int
+    d_times: int
+    d_out: int
+    d_stats: int = 0
+    dt_one: float = INITIAL_STEP_SENTINEL
+    span_hint: float = 0.0
+
+    def to_c(self) -> CJob:
+        return CJob(self.method, self.potential, self.n, self.nt, self.rtol_ln, self.atol_ln,
+                    self.dt_one, self.span_hint, self.d_state0, self.d_times, self.d_out,
+                    self.d_stats)
+
+
+def ragged_device(jobs: list[DeviceJob], potentials: list, precision: int = PRECISION_FP64,
+                  block_size: int = 0, max_attempts: int = 0, stream: int = 0,
+                  device: int = 0) -> None:
+    """drift_ragged_device: every job to completion, one persistent launch per
+    method (returns when done)."""
+    s = session(device)
+    arr = (CJob * len(jobs))(*[j.to_c() for j in jobs])
+    pots = (CPotential * len(potentials))(
+        *[(p.to_c() if p is not None else PotentialSpec.galpy_kepler().to_c()) for p in potentials])
+    opts = CLaunchOptions(precision, block_size, 0, max_attempts)
+    check(s.lib.drift_ragged_device(s.handle, arr, len(jobs), pots, len(potentials),
+                                    ctypes.byref(opts), ctypes.c_void_p(stream)))
 
 
 def force_device(potential: PotentialSpec | None, d_pos: int, d_acc: int, n: int,

@@ -299,59 +299,19 @@ __global__ void __launch_bounds__(kDefaultBlock) force_kernel(ForceArgs a) {
 // ---------------------------------------------------------------------------
 
 namespace {
-
 thread_local std::string g_last_error;
-
-struct TableEntry {
-    drift_potential key;
-    PotParams params;
-    double* d_table64 = nullptr;
-    float* d_table32 = nullptr;
-};
-
-struct DeviceBuffer {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-};
-
 }  // namespace
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 }  // namespace drift
 
-struct drift_session {
-    int device = 0;
-    int cu_count = 0;
-    hipStream_t stream = nullptr;
-    std::vector<drift::TableEntry> tables;
-    drift::DeviceBuffer state0, times, out, stats;
-    // ring of work-queue counters for persistent launches (one per launch in
-    // flight; launches further apart than the ring reuse a counter only after
-    // stream order has retired its previous user)
-    unsigned long long* queues = nullptr;
-    unsigned queue_next = 0;
-};
-
 namespace drift {
-namespace {
-
-#define DRIFT_HIP_TRY(expr)                                                              \
-    do {                                                                                  \
-        hipError_t _e = (expr);                                                           \
-        if (_e != hipSuccess) {                                                           \
-            set_last_error(std::string("HIP error: ") + hipGetErrorString(_e) + " in " + \
-                           #expr);                                                       \
-            return _e == hipErrorOutOfMemory ? DRIFT_ERR_OUT_OF_MEMORY : DRIFT_ERR_DEVICE; \
-        }                                                                                 \
-    } while (0)
 
 int fail(int code, const std::string& msg) {
     set_last_error(msg);
     return code;
 }
-
-bool finite(double x) { return std::isfinite(x); }
 
 int ensure_buffer(DeviceBuffer& b, size_t bytes) {
     if (b.bytes >= bytes) return DRIFT_OK;
@@ -363,9 +323,15 @@ int ensure_buffer(DeviceBuffer& b, size_t bytes) {
     return DRIFT_OK;
 }
 
+namespace {
+
+bool finite(double x) { return std::isfinite(x); }
+
 bool same_potential(const drift_potential& a, const drift_potential& b) {
     return std::memcmp(&a, &b, sizeof(drift_potential)) == 0;
 }
+
+}  // namespace
 
 int validate_potential(const drift_potential& p) {
     if (p.reserved != 0) return fail(DRIFT_ERR_INVALID_ARGUMENT, "drift_potential.reserved must be 0");
@@ -445,6 +411,8 @@ int resolve_potential(drift_session* s, const drift_potential* potential, PotPar
     }
     return DRIFT_OK;
 }
+
+namespace {
 
 using PlainKernel = void (*)(PlainArgs);
 
@@ -634,6 +602,10 @@ int drift_session_destroy(drift_session* s) {
     for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats})
         if (b->ptr) (void)hipFree(b->ptr);
     if (s->queues) (void)hipFree(s->queues);
+    for (auto& r : s->ragged) {
+        if (r.desc.ptr) (void)hipFree(r.desc.ptr);
+        if (r.queue) (void)hipFree(r.queue);
+    }
     (void)hipStreamDestroy(s->stream);
     delete s;
     return DRIFT_OK;

@@ -2,12 +2,69 @@
 // of libdrift_hip.so.
 #pragma once
 
+#include <hip/hip_runtime_api.h>
+
 #include <string>
+#include <vector>
 
 #include "../../include/drift_hip.h"
 #include "drift_device.hpp"
 
 namespace drift {
+
+struct TableEntry {
+    drift_potential key;
+    PotParams params;
+    double* d_table64 = nullptr;
+    float* d_table32 = nullptr;
+};
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace drift
+
+#include "ragged.hpp"
+
+struct drift_session {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    std::vector<drift::TableEntry> tables;
+    drift::DeviceBuffer state0, times, out, stats;
+    // ring of work-queue counters for persistent launches (one per launch in
+    // flight; launches further apart than the ring reuse a counter only after
+    // stream order has retired its previous user)
+    unsigned long long* queues = nullptr;
+    unsigned queue_next = 0;
+    // descriptor buffers of drift_ragged_device, one per method
+    drift::RaggedScratch ragged[2];
+};
+
+#define DRIFT_HIP_TRY(expr)                                                              \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            drift::set_last_error(std::string("HIP error: ") + hipGetErrorString(_e) +    \
+                                  " in " + #expr);                                        \
+            return _e == hipErrorOutOfMemory ? DRIFT_ERR_OUT_OF_MEMORY : DRIFT_ERR_DEVICE; \
+        }                                                                                 \
+    } while (0)
+
+namespace drift {
+
+constexpr unsigned kQueueRing = 64;
+
+// Error helpers (drift_hip.hip).
+int fail(int code, const std::string& msg);
+int ensure_buffer(DeviceBuffer& b, size_t bytes);
+int validate_potential(const drift_potential& p);
+// Kernel parameters (and the cached device table for MW) of a potential
+// descriptor; NULL selects the unit galpy Kepler force.
+int resolve_potential(drift_session* s, const drift_potential* potential, PotParams& params,
+                      const void** table64, const void** table32);
 
 // Milky-Way spherical table (spherical_table.cpp).
 void build_mw_params(const drift_potential& p, PotParams& out);

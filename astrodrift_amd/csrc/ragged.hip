@@ -1,0 +1,523 @@
+// ragged.hip -- ragged batches: many independent jobs (different particle
+// counts, potentials, time grids and tolerances) in one persistent launch per
+// integration method, with resumable per-orbit state for chunked execution.
+//
+// Work items are (job, orbit) pairs numbered job by job.  Lanes claim items
+// from a global queue (wavefront-ballot aggregation, as the persistent plain
+// schedule), locate their job by binary search over the item offsets, and
+// integrate the orbit until it has emitted the job's stop sample or finished.
+// A paused orbit's complete integrator state is saved to a per-job resident
+// buffer (SoA: word w of orbit i at resident[w * n + i]) and restored bit for
+// bit by the next chunk, so chunking never changes a result.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "drift_internal.hpp"
+#include "forces.hpp"
+#include "integrators.hpp"
+#include "ragged.hpp"
+
+namespace drift {
+
+// ---------------------------------------------------------------------------
+// Device side
+// ---------------------------------------------------------------------------
+
+// Force of a lane's current job, dispatched at run time (jobs in one launch
+// may use different potentials).  c0..c2 are the kind's constants.
+template <class FT>
+struct LaneForce {
+    int32_t kind;
+    FT c0, c1, c2;
+    const FT* table;
+    const PotParams* slow;
+    __device__ __forceinline__ void operator()(const double* q, double* a) const {
+        switch (kind) {
+            case POT_GALPY_KEPLER: GalpyKeplerForce<FT>{}(q, a); break;
+            case POT_KEPLER: KeplerForce<FT>{c0}(q, a); break;
+            case POT_PLUMMER: PlummerForce<FT>{c0, c1}(q, a); break;
+            default: MwForce<FT>{slow, table, c0, c1, c2}(q, a); break;
+        }
+    }
+};
+
+template <class FT>
+__device__ __forceinline__ LaneForce<FT> lane_force(const RaggedArgs& a, int32_t pot) {
+    const PotParams* p = a.pots + pot;
+    LaneForce<FT> f;
+    f.kind = p->kind;
+    f.slow = p;
+    if (p->kind == POT_MW) {
+        f.c0 = FT(p->disk_amp);
+        f.c1 = FT(p->disk_a);
+        f.c2 = FT(p->disk_b2);
+    } else {
+        f.c0 = FT(p->amp);
+        f.c1 = FT(p->b2);
+        f.c2 = FT(0);
+    }
+    if constexpr (sizeof(FT) == 8)
+        f.table = static_cast<const FT*>(a.tables64[pot]);
+    else
+        f.table = static_cast<const FT*>(a.tables32[pot]);
+    return f;
+}
+
+__device__ __forceinline__ int32_t find_job(const RaggedJob* jobs, int32_t njobs,
+                                            unsigned long long slot) {
+    int32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if ((unsigned long long)jobs[mid].item_begin <= slot) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ double pack2(int32_t lo, int32_t hi) {
+    return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) |
+                                            (unsigned long long)(uint32_t)lo));
+}
+__device__ __forceinline__ void unpack2(double w, int32_t& lo, int32_t& hi) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(w);
+    lo = (int32_t)(uint32_t)(b & 0xffffffffull);
+    hi = (int32_t)(uint32_t)(b >> 32);
+}
+
+struct Resident {
+    double* base;
+    int64_t n;
+    int64_t i;
+    __device__ __forceinline__ double& w(int k) const { return base[(int64_t)k * n + i]; }
+};
+
+__device__ __forceinline__ void save(const Resident& r, const Dop853Lane& L, int64_t phase) {
+#pragma unroll
+    for (int k = 0; k < DIM; k++) {
+        r.w(k) = L.y[k];
+        r.w(6 + k) = L.k1[k];
+    }
+    r.w(12) = L.h;
+    r.w(13) = L.t_cur;
+    r.w(14) = L.t_old;
+    r.w(15) = pack2(L.ii, L.reject);
+    r.w(16) = pack2((int32_t)L.accepted, (int32_t)L.rejected);
+    r.w(17) = pack2((int32_t)L.dense, L.status);
+    r.w(kResidentPhaseWord) = __longlong_as_double(phase);
+}
+
+__device__ __forceinline__ void restore(const Resident& r, Dop853Lane& L) {
+#pragma unroll
+    for (int k = 0; k < DIM; k++) {
+        L.y[k] = r.w(k);
+        L.k1[k] = r.w(6 + k);
+    }
+    L.h = r.w(12);
+    L.t_cur = r.w(13);
+    L.t_old = r.w(14);
+    int32_t a, b;
+    unpack2(r.w(15), L.ii, L.reject);
+    unpack2(r.w(16), a, b);
+    L.accepted = (uint32_t)a;
+    L.rejected = (uint32_t)b;
+    unpack2(r.w(17), a, L.status);
+    L.dense = (uint32_t)a;
+}
+
+__device__ __forceinline__ void save(const Resident& r, const Dopr54Lane& L, int64_t phase) {
+#pragma unroll
+    for (int k = 0; k < DIM; k++) {
+        r.w(k) = L.y[k];
+        r.w(6 + k) = L.a1[k];
+    }
+    r.w(12) = L.to;
+    r.w(13) = L.dt_one;
+    r.w(14) = L.init_to;
+    r.w(15) = pack2(L.ii, L.err);
+    r.w(16) = pack2((int32_t)L.accepted, (int32_t)L.rejected);
+    r.w(17) = L.init_dt_one;
+    r.w(18) = pack2(L.status, 0);
+    r.w(kResidentPhaseWord) = __longlong_as_double(phase);
+}
+
+__device__ __forceinline__ void restore(const Resident& r, Dopr54Lane& L) {
+#pragma unroll
+    for (int k = 0; k < DIM; k++) {
+        L.y[k] = r.w(k);
+        L.a1[k] = r.w(6 + k);
+    }
+    L.to = r.w(12);
+    L.dt_one = r.w(13);
+    L.init_to = r.w(14);
+    int32_t a, b;
+    unpack2(r.w(15), L.ii, L.err);
+    unpack2(r.w(16), a, b);
+    L.accepted = (uint32_t)a;
+    L.rejected = (uint32_t)b;
+    L.init_dt_one = r.w(17);
+    unpack2(r.w(18), L.status, a);
+}
+
+struct OutSinkR {
+    double* out;
+    int64_t n;
+    int64_t idx;
+    __device__ __forceinline__ void emit(int32_t s, const double* y) const {
+        double2* p = reinterpret_cast<double2*>(out + ((int64_t)s * n + idx) * DIM);
+        p[0] = make_double2(y[0], y[1]);
+        p[1] = make_double2(y[2], y[3]);
+        p[2] = make_double2(y[4], y[5]);
+    }
+};
+
+__device__ __forceinline__ bool claim_item(bool need, unsigned long long* queue,
+                                           unsigned long long items,
+                                           unsigned long long& slot) {
+    const unsigned long long want = __ballot(need);
+    if (want == 0) return true;
+    const int lane = __lane_id();
+    const int leader = __ffsll(want) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(queue, (unsigned long long)__popcll(want));
+    base = __shfl(base, leader);
+    if (!need) return true;
+    const unsigned long long below = lane == 0 ? 0ull : (want & (~0ull >> (64 - lane)));
+    slot = base + (unsigned long long)__popcll(below);
+    return slot < items;
+}
+
+template <class FT>
+__global__ void __launch_bounds__(256) ragged_dop853_kernel(RaggedArgs a) {
+    Dop853Lane L;
+    Dop853Job J;
+    LaneForce<FT> f;
+    OutSinkR sink{nullptr, 0, 0};
+    Resident res{nullptr, 0, 0};
+    drift_orbit_stats* stats = nullptr;
+    unsigned int* done_count = nullptr;
+    int32_t stop = 0;
+    bool need = true;
+    for (;;) {
+        unsigned long long slot = 0;
+        if (!claim_item(need, a.queue, (unsigned long long)a.items, slot)) break;
+        if (need) {
+            const RaggedJob& d = a.jobs[find_job(a.jobs, a.njobs, slot)];
+            const int64_t orbit = (int64_t)(slot - (unsigned long long)d.item_begin);
+            res = Resident{d.resident, d.n, orbit};
+            const int64_t phase = d.resident ? __double_as_longlong(res.w(kResidentPhaseWord))
+                                             : kOrbitNew;
+            if (phase == kOrbitDone) continue;
+            J = make_dop853_job(d.times, d.nt, d.rtol, d.atol, a.max_attempts, d.times[0],
+                                d.times[d.nt - 1]);
+            f = lane_force<FT>(a, d.pot);
+            sink = OutSinkR{d.out, d.n, orbit};
+            stats = d.stats;
+            done_count = d.done_count;
+            stop = d.stop_sample;
+            if (phase == kOrbitRunning) {
+                restore(res, L);
+            } else {
+                double y0[DIM];
+                const double2* p = reinterpret_cast<const double2*>(d.state0 + orbit * DIM);
+                double2 u = p[0], v = p[1], w = p[2];
+                y0[0] = u.x; y0[1] = u.y; y0[2] = v.x; y0[3] = v.y; y0[4] = w.x; y0[5] = w.y;
+                dop853_init(L, y0, J.times[0], J, f, sink);
+            }
+            need = false;
+        }
+        const bool done = dop853_attempt(L, J, f, sink);
+        if (done || L.ii >= stop) {
+            if (res.base) save(res, L, done ? kOrbitDone : kOrbitRunning);
+            if (done) {
+                if (stats) {
+                    drift_orbit_stats st;
+                    st.accepted = L.accepted;
+                    st.rejected = L.rejected;
+                    st.dense_steps = L.dense;
+                    st.status = L.status;
+                    stats[sink.idx] = st;
+                }
+                if (done_count) atomicAdd(done_count, 1u);
+            }
+            need = true;
+        }
+    }
+}
+
+template <class FT>
+__global__ void __launch_bounds__(256) ragged_dopr54_kernel(RaggedArgs a) {
+    Dopr54Lane L;
+    Dopr54Job J;
+    LaneForce<FT> f;
+    OutSinkR sink{nullptr, 0, 0};
+    Resident res{nullptr, 0, 0};
+    drift_orbit_stats* stats = nullptr;
+    unsigned int* done_count = nullptr;
+    int32_t stop = 0;
+    bool need = true;
+#pragma unroll
+    for (int k = 0; k < 6; k++) J.bound2[k] = a.bound2[k];
+    for (;;) {
+        unsigned long long slot = 0;
+        if (!claim_item(need, a.queue, (unsigned long long)a.items, slot)) break;
+        bool done = false;
+        if (need) {
+            const RaggedJob& d = a.jobs[find_job(a.jobs, a.njobs, slot)];
+            const int64_t orbit = (int64_t)(slot - (unsigned long long)d.item_begin);
+            res = Resident{d.resident, d.n, orbit};
+            const int64_t phase = d.resident ? __double_as_longlong(res.w(kResidentPhaseWord))
+                                             : kOrbitNew;
+            if (phase == kOrbitDone) continue;
+            J.times = d.times;
+            J.nt = d.nt;
+            J.max_attempts = a.max_attempts;
+            J.dt = d.times[1] - d.times[0];
+            J.rtol = d.rtol;
+            J.atol = d.atol;
+            J.dt_one_init = d.dt_one_init;
+            f = lane_force<FT>(a, d.pot);
+            sink = OutSinkR{d.out, d.n, orbit};
+            stats = d.stats;
+            done_count = d.done_count;
+            stop = d.stop_sample;
+            if (phase == kOrbitRunning) {
+                restore(res, L);
+            } else {
+                double y0[DIM];
+                const double2* p = reinterpret_cast<const double2*>(d.state0 + orbit * DIM);
+                double2 u = p[0], v = p[1], w = p[2];
+                y0[0] = u.x; y0[1] = u.y; y0[2] = v.x; y0[3] = v.y; y0[4] = w.x; y0[5] = w.y;
+                dopr54_init(L, y0, d.times[0], J, f, sink);
+                done = L.ii >= J.nt - 1;
+            }
+            need = false;
+        }
+        if (!done) done = dopr54_attempt(L, J, f, sink);
+        if (done || L.ii >= stop) {
+            if (res.base) save(res, L, done ? kOrbitDone : kOrbitRunning);
+            if (done) {
+                if (stats) {
+                    drift_orbit_stats st;
+                    st.accepted = L.accepted;
+                    st.rejected = L.rejected;
+                    st.dense_steps = 0;
+                    st.status = L.err | L.status;
+                    stats[sink.idx] = st;
+                }
+                if (done_count) atomicAdd(done_count, 1u);
+            }
+            need = true;
+        }
+    }
+}
+
+__global__ void ragged_reset_queue(unsigned long long* queue) { *queue = 0ull; }
+
+// Marks every orbit of a resident buffer "not started" (a fill kernel, so the
+// sequence stays valid inside a captured graph).
+__global__ void ragged_clear_phase(double* resident, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) resident[(int64_t)kResidentPhaseWord * n + i] = __longlong_as_double(kOrbitNew);
+}
+
+__global__ void ragged_clear_u32(unsigned int* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
+                  const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
+                  const std::vector<const void*>& t32, const drift_launch_options& opt,
+                  hipStream_t stream, RaggedScratch& scratch) {
+    if (njobs == 0) return DRIFT_OK;
+    const int64_t items = host_jobs[njobs - 1].item_begin + host_jobs[njobs - 1].n;
+    if (items == 0) return DRIFT_OK;
+    // one contiguous upload: jobs | pots | tables64 | tables32
+    const size_t jobs_bytes = sizeof(RaggedJob) * (size_t)njobs;
+    const size_t pots_bytes = sizeof(PotParams) * pots.size();
+    const size_t tab_bytes = sizeof(void*) * pots.size();
+    const size_t total = jobs_bytes + pots_bytes + 2 * tab_bytes;
+    int rc = ensure_buffer(scratch.desc, total);
+    if (rc != DRIFT_OK) return rc;
+    scratch.staging.resize(total);
+    char* h = scratch.staging.data();
+    std::memcpy(h, host_jobs, jobs_bytes);
+    std::memcpy(h + jobs_bytes, pots.data(), pots_bytes);
+    std::memcpy(h + jobs_bytes + pots_bytes, t64.data(), tab_bytes);
+    std::memcpy(h + jobs_bytes + pots_bytes + tab_bytes, t32.data(), tab_bytes);
+    DRIFT_HIP_TRY(hipMemcpyAsync(scratch.desc.ptr, h, total, hipMemcpyHostToDevice, stream));
+    if (!scratch.queue) DRIFT_HIP_TRY(hipMalloc(&scratch.queue, sizeof(unsigned long long)));
+
+    char* d = static_cast<char*>(scratch.desc.ptr);
+    RaggedArgs a{};
+    a.jobs = reinterpret_cast<const RaggedJob*>(d);
+    a.njobs = njobs;
+    a.items = items;
+    a.pots = reinterpret_cast<const PotParams*>(d + jobs_bytes);
+    a.tables64 = reinterpret_cast<const void* const*>(d + jobs_bytes + pots_bytes);
+    a.tables32 = reinterpret_cast<const void* const*>(d + jobs_bytes + pots_bytes + tab_bytes);
+    a.queue = scratch.queue;
+    a.max_attempts = opt.max_attempts ? opt.max_attempts : (1u << 22);
+    dopr54_bounds(a.bound2);
+
+    using K = void (*)(RaggedArgs);
+    const bool mixed = opt.precision == DRIFT_PRECISION_MIXED;
+    K k;
+    if (method == DRIFT_METHOD_DOP853)
+        k = mixed ? ragged_dop853_kernel<float> : ragged_dop853_kernel<double>;
+    else
+        k = mixed ? ragged_dopr54_kernel<float> : ragged_dopr54_kernel<double>;
+    const int block = opt.block_size ? opt.block_size : 256;
+    int per_cu = 0;
+    DRIFT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0));
+    int64_t blocks = (int64_t)std::max(per_cu, 1) * std::max(s->cu_count, 1);
+    blocks = std::min<int64_t>(blocks, (items + block - 1) / block);
+    hipLaunchKernelGGL(ragged_reset_queue, dim3(1), dim3(1), 0, stream, a.queue);
+    DRIFT_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, stream, a);
+    DRIFT_HIP_TRY(hipGetLastError());
+    return DRIFT_OK;
+}
+
+int ragged_clear(double* resident, int64_t n, unsigned int* done_count, hipStream_t stream) {
+    if (resident && n > 0) {
+        hipLaunchKernelGGL(ragged_clear_phase, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           stream, resident, n);
+        DRIFT_HIP_TRY(hipGetLastError());
+    }
+    if (done_count) {
+        hipLaunchKernelGGL(ragged_clear_u32, dim3(1), dim3(64), 0, stream, done_count, 1);
+        DRIFT_HIP_TRY(hipGetLastError());
+    }
+    return DRIFT_OK;
+}
+
+// Estimated relative cost of one orbit of a job: DOP853 steps scale as
+// rtol^(-1/8), DOPR54 as rtol^(-1/5), both linearly with the span (the
+// caller's span_hint; the grid itself is in device memory).
+double job_cost(const drift_job& j) {
+    const double e = j.method == DRIFT_METHOD_DOP853 ? -1.0 / 8.0 : -1.0 / 5.0;
+    return std::exp(e * std::min(j.rtol, j.atol)) * std::fabs(j.span_hint > 0 ? j.span_hint : 1.0);
+}
+
+int validate_jobs(const drift_job* jobs, size_t njobs, size_t npots) {
+    for (size_t k = 0; k < njobs; ++k) {
+        const drift_job& j = jobs[k];
+        const std::string where = "job " + std::to_string(k) + ": ";
+        if (j.method != DRIFT_METHOD_DOPR54 && j.method != DRIFT_METHOD_DOP853)
+            return fail(DRIFT_ERR_INVALID_ARGUMENT, where + "unknown method");
+        if (j.potential < 0 || (size_t)j.potential >= npots)
+            return fail(DRIFT_ERR_INVALID_ARGUMENT, where + "potential index out of range");
+        if (j.nt < 2 || j.nt > (size_t)INT32_MAX)
+            return fail(DRIFT_ERR_INVALID_ARGUMENT, where + "nt must be in [2, 2^31)");
+        if (j.n > 0 && (!j.d_state0 || !j.d_times || !j.d_state_out))
+            return fail(DRIFT_ERR_INVALID_ARGUMENT, where + "NULL buffer");
+        if ((reinterpret_cast<uintptr_t>(j.d_state0) | reinterpret_cast<uintptr_t>(j.d_state_out)) &
+            15u)
+            return fail(DRIFT_ERR_INVALID_ARGUMENT, where + "buffers must be 16-byte aligned");
+        if (!std::isfinite(j.rtol) || !std::isfinite(j.atol))
+            return fail(DRIFT_ERR_INVALID_ARGUMENT, where + "rtol/atol (logs) must be finite");
+    }
+    return DRIFT_OK;
+}
+
+int resolve_pots(drift_session* s, const drift_potential* potentials, size_t npots,
+                 std::vector<PotParams>& pots, std::vector<const void*>& t64,
+                 std::vector<const void*>& t32) {
+    pots.resize(npots);
+    t64.assign(npots, nullptr);
+    t32.assign(npots, nullptr);
+    for (size_t k = 0; k < npots; ++k) {
+        int rc = resolve_potential(s, potentials ? &potentials[k] : nullptr, pots[k], &t64[k],
+                                   &t32[k]);
+        if (rc != DRIFT_OK) return rc;
+    }
+    return DRIFT_OK;
+}
+
+// Job order inside a launch: grouped by potential kind (lanes of a wavefront
+// then mostly evaluate one force), costliest first within a group so the
+// queue's tail is made of short orbits.
+std::vector<size_t> ragged_order(const drift_job* jobs, size_t njobs,
+                                 const std::vector<PotParams>& pots, int method) {
+    std::vector<size_t> idx;
+    for (size_t k = 0; k < njobs; ++k)
+        if (jobs[k].method == method && jobs[k].n > 0) idx.push_back(k);
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        const int ka = pots[jobs[a].potential].kind, kb = pots[jobs[b].potential].kind;
+        if (ka != kb) return ka > kb;
+        return job_cost(jobs[a]) > job_cost(jobs[b]);
+    });
+    return idx;
+}
+
+}  // namespace drift
+
+using namespace drift;
+
+extern "C" int drift_ragged_device(drift_session* session, const drift_job* jobs, size_t njobs,
+                                   const drift_potential* potentials, size_t npotentials,
+                                   const drift_launch_options* options, void* stream) {
+    if (!session) return fail(DRIFT_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (njobs == 0) return DRIFT_OK;
+    if (!jobs) return fail(DRIFT_ERR_INVALID_ARGUMENT, "jobs is NULL");
+    if (npotentials == 0) return fail(DRIFT_ERR_INVALID_ARGUMENT, "no potentials");
+    int rc = validate_jobs(jobs, njobs, npotentials);
+    if (rc != DRIFT_OK) return rc;
+    drift_launch_options opt{};
+    if (options) opt = *options;
+    if (opt.precision != DRIFT_PRECISION_FP64 && opt.precision != DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "unknown precision mode");
+    if (opt.block_size && (opt.block_size < 64 || opt.block_size > 256 || opt.block_size % 64))
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
+    DRIFT_HIP_TRY(hipSetDevice(session->device));
+    std::vector<PotParams> pots;
+    std::vector<const void*> t64, t32;
+    rc = resolve_pots(session, potentials, npotentials, pots, t64, t32);
+    if (rc != DRIFT_OK) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : session->stream;
+    for (int method : {DRIFT_METHOD_DOP853, DRIFT_METHOD_DOPR54}) {
+        std::vector<size_t> order = ragged_order(jobs, njobs, pots, method);
+        std::vector<RaggedJob> rj;
+        int64_t begin = 0;
+        for (size_t k : order) {
+            const drift_job& j = jobs[k];
+            if (opt.precision == DRIFT_PRECISION_MIXED && std::exp(j.rtol) < 1e-7 * (1 - 1e-9))
+                return fail(DRIFT_ERR_UNSUPPORTED,
+                            "mixed precision (FP32 stage forces) requires rtol >= 1e-7");
+            RaggedJob r{};
+            r.state0 = j.d_state0;
+            r.times = j.d_times;
+            r.out = j.d_state_out;
+            r.stats = j.d_stats;
+            r.resident = nullptr;
+            r.done_count = nullptr;
+            r.n = (int64_t)j.n;
+            r.item_begin = begin;
+            r.nt = (int32_t)j.nt;
+            r.method = j.method;
+            r.stop_sample = (int32_t)j.nt - 1;
+            r.pot = j.potential;
+            r.rtol = std::exp(j.rtol);
+            r.atol = std::exp(j.atol);
+            r.dt_one_init = j.dt_one_init;
+            begin += r.n;
+            rj.push_back(r);
+        }
+        rc = ragged_launch(session, rj.data(), (int32_t)rj.size(), method, pots, t64, t32, opt, st,
+                           session->ragged[method]);
+        if (rc != DRIFT_OK) return rc;
+    }
+    // the session's descriptor buffers are reused by the next call: wait
+    DRIFT_HIP_TRY(hipStreamSynchronize(st));
+    return DRIFT_OK;
+}

@@ -1,0 +1,68 @@
+// ragged.hpp -- descriptors shared by the ragged kernels and their host code.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/drift_hip.h"
+#include "drift_device.hpp"
+
+struct drift_session;
+
+namespace drift {
+
+// Resident (resumable) state: kResidentWords 8-byte words per orbit, SoA.
+constexpr int kResidentWords = 20;
+constexpr int kResidentPhaseWord = 19;
+constexpr int64_t kOrbitNew = 0;
+constexpr int64_t kOrbitRunning = 1;
+constexpr int64_t kOrbitDone = 2;
+
+// One job as the kernels see it.
+struct RaggedJob {
+    const double* state0;       // (n, 6)
+    const double* times;        // (nt,)
+    double* out;                // (nt, n, 6)
+    drift_orbit_stats* stats;   // (n,) or NULL
+    double* resident;           // (kResidentWords, n) or NULL = run to completion
+    unsigned int* done_count;   // orbits finished, or NULL
+    int64_t n;
+    int64_t item_begin;         // first work item of this job
+    int32_t nt;
+    int32_t method;
+    int32_t stop_sample;        // pause an orbit once it has emitted this sample
+    int32_t pot;                // index into the launch's potentials
+    double rtol, atol;          // linear tolerances
+    double dt_one_init;         // DOPR54
+};
+
+struct RaggedArgs {
+    const RaggedJob* jobs;
+    int32_t njobs;
+    uint32_t max_attempts;
+    int64_t items;
+    const PotParams* pots;
+    const void* const* tables64;
+    const void* const* tables32;
+    unsigned long long* queue;
+    double bound2[6];
+};
+
+struct RaggedScratch {
+    DeviceBuffer desc;
+    std::vector<char> staging;
+    unsigned long long* queue = nullptr;
+};
+
+int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
+                  const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
+                  const std::vector<const void*>& t32, const drift_launch_options& opt,
+                  hipStream_t stream, RaggedScratch& scratch);
+int ragged_clear(double* resident, int64_t n, unsigned int* done_count, hipStream_t stream);
+int resolve_pots(drift_session* s, const drift_potential* potentials, size_t npots,
+                 std::vector<PotParams>& pots, std::vector<const void*>& t64,
+                 std::vector<const void*>& t32);
+
+}  // namespace drift

@@ -1,0 +1,384 @@
+// serving.cpp -- continuous-batching request loop over the ragged kernels.
+//
+// A server owns a session (device, stream, MW table cache) and one worker
+// thread.  Submitted requests queue first-come first-served; the worker admits
+// them while their device footprint fits the HBM budget, then runs chunk
+// launches: each advances every admitted request by up to chunk_samples
+// output samples (one persistent ragged launch per integration method), the
+// orbits' integrator state resident in HBM between chunks.  A request whose
+// orbits have all finished is copied to its host buffer and its device memory
+// released, which admits the next requests at the following chunk boundary.
+// So requests that arrive while others run join the GPU work within one
+// chunk, and every result is independent of what it shared the GPU with.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "drift_internal.hpp"
+#include "ragged.hpp"
+
+namespace drift {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t ticket = 0;
+    drift_request req{};
+    std::vector<double> state0, times;
+    drift_request_info info{};
+    uint64_t bytes = 0;
+    int32_t cursor = 0;
+    double* d_state0 = nullptr;
+    double* d_times = nullptr;
+    double* d_out = nullptr;
+    double* d_resident = nullptr;
+    drift_orbit_stats* d_stats = nullptr;
+    unsigned int* d_done = nullptr;
+    unsigned int host_done = 0;
+};
+
+uint64_t footprint(const drift_request& r) {
+    const uint64_t n = r.n, nt = r.nt;
+    return 8ull * (6 * n + nt + 6 * n * nt + (uint64_t)kResidentWords * n) + 16ull * n + 256;
+}
+
+void free_device(Request& r) {
+    for (void* p : {(void*)r.d_state0, (void*)r.d_times, (void*)r.d_out, (void*)r.d_resident,
+                    (void*)r.d_stats, (void*)r.d_done})
+        if (p) (void)hipFree(p);
+    r.d_state0 = r.d_times = r.d_out = r.d_resident = nullptr;
+    r.d_stats = nullptr;
+    r.d_done = nullptr;
+}
+
+}  // namespace
+}  // namespace drift
+
+struct drift_server {
+    drift_session* session = nullptr;
+    drift_server_options opt{};
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<drift::Request>> pending;
+    std::map<uint64_t, std::shared_ptr<drift::Request>> requests;
+    uint64_t next_ticket = 1;
+    uint64_t used_bytes = 0;
+    bool stopping = false;
+    drift::Clock::time_point t0;
+    drift::RaggedScratch scratch[2];
+};
+
+namespace drift {
+namespace {
+
+double now_s(const drift_server* S) {
+    return std::chrono::duration<double>(Clock::now() - S->t0).count();
+}
+
+void complete(drift_server* S, Request& r, int state, int status) {
+    free_device(r);
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->used_bytes -= r.bytes;
+    r.info.state = state;
+    r.info.status = status;
+    r.info.complete_s = now_s(S);
+    S->cv_done.notify_all();
+}
+
+// Allocates the request's device buffers and uploads its inputs.
+int admit(drift_server* S, Request& r, hipStream_t st) {
+    const size_t n = r.req.n, nt = r.req.nt;
+    DRIFT_HIP_TRY(hipMalloc(&r.d_times, nt * sizeof(double)));
+    DRIFT_HIP_TRY(hipMalloc(&r.d_done, sizeof(unsigned int)));
+    if (n > 0) {
+        DRIFT_HIP_TRY(hipMalloc(&r.d_state0, n * DIM * sizeof(double)));
+        DRIFT_HIP_TRY(hipMalloc(&r.d_out, nt * n * DIM * sizeof(double)));
+        DRIFT_HIP_TRY(hipMalloc(&r.d_resident, (size_t)kResidentWords * n * sizeof(double)));
+        DRIFT_HIP_TRY(hipMalloc(&r.d_stats, n * sizeof(drift_orbit_stats)));
+        DRIFT_HIP_TRY(hipMemcpyAsync(r.d_state0, r.state0.data(), n * DIM * sizeof(double),
+                                     hipMemcpyHostToDevice, st));
+    }
+    DRIFT_HIP_TRY(hipMemcpyAsync(r.d_times, r.times.data(), nt * sizeof(double),
+                                 hipMemcpyHostToDevice, st));
+    int rc = ragged_clear(r.d_resident, (int64_t)n, r.d_done, st);
+    if (rc != DRIFT_OK) return rc;
+    std::lock_guard<std::mutex> lk(S->mu);
+    r.info.state = DRIFT_REQUEST_RUNNING;
+    r.info.admit_s = now_s(S);
+    return DRIFT_OK;
+}
+
+// Copies a finished request to its host buffers.
+int finish(drift_server* S, Request& r, hipStream_t st, int& status) {
+    const size_t n = r.req.n, nt = r.req.nt;
+    status = DRIFT_OK;
+    if (n == 0) return DRIFT_OK;
+    std::vector<drift_orbit_stats> own;
+    drift_orbit_stats* stats = r.req.stats;
+    if (!stats) {
+        own.resize(n);
+        stats = own.data();
+    }
+    DRIFT_HIP_TRY(hipMemcpyAsync(r.req.state_out, r.d_out, nt * n * DIM * sizeof(double),
+                                 hipMemcpyDeviceToHost, st));
+    DRIFT_HIP_TRY(hipMemcpyAsync(stats, r.d_stats, n * sizeof(drift_orbit_stats),
+                                 hipMemcpyDeviceToHost, st));
+    DRIFT_HIP_TRY(hipStreamSynchronize(st));
+    uint64_t acc = 0;
+    for (size_t i = 0; i < n; ++i) {
+        acc += stats[i].accepted;
+        if (r.req.method == DRIFT_METHOD_DOPR54 && (stats[i].status & DRIFT_STATUS_CODE_MASK))
+            status = DRIFT_ERR_INTEGRATION;
+    }
+    r.info.accepted_steps = acc;
+    return DRIFT_OK;
+}
+
+// One chunk over every active request.
+int run_chunk(drift_server* S, std::vector<std::shared_ptr<Request>>& active, hipStream_t st) {
+    // distinct potentials of this chunk
+    std::vector<drift_potential> keys;
+    std::vector<int32_t> pot_of(active.size());
+    for (size_t k = 0; k < active.size(); ++k) {
+        const drift_potential& p = active[k]->req.potential;
+        auto it = std::find_if(keys.begin(), keys.end(), [&](const drift_potential& q) {
+            return std::memcmp(&p, &q, sizeof(p)) == 0;
+        });
+        pot_of[k] = (int32_t)(it - keys.begin());
+        if (it == keys.end()) keys.push_back(p);
+    }
+    std::vector<PotParams> pots;
+    std::vector<const void*> t64, t32;
+    int rc = resolve_pots(S->session, keys.data(), keys.size(), pots, t64, t32);
+    if (rc != DRIFT_OK) return rc;
+    drift_launch_options lo{};
+    lo.precision = S->opt.precision;
+    lo.block_size = S->opt.block_size;
+    lo.max_attempts = S->opt.max_attempts;
+    const int32_t chunk = S->opt.chunk_samples > 0 ? S->opt.chunk_samples : 8;
+    for (int method : {DRIFT_METHOD_DOP853, DRIFT_METHOD_DOPR54}) {
+        std::vector<size_t> idx;
+        for (size_t k = 0; k < active.size(); ++k)
+            if (active[k]->req.method == method && active[k]->req.n > 0) idx.push_back(k);
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](size_t a, size_t b) { return pots[pot_of[a]].kind > pots[pot_of[b]].kind; });
+        std::vector<RaggedJob> jobs;
+        int64_t begin = 0;
+        for (size_t k : idx) {
+            Request& r = *active[k];
+            const int32_t last = (int32_t)r.req.nt - 1;
+            r.cursor = std::min(last, r.cursor + chunk);
+            RaggedJob j{};
+            j.state0 = r.d_state0;
+            j.times = r.d_times;
+            j.out = r.d_out;
+            j.stats = r.d_stats;
+            j.resident = r.d_resident;
+            j.done_count = r.d_done;
+            j.n = (int64_t)r.req.n;
+            j.item_begin = begin;
+            j.nt = (int32_t)r.req.nt;
+            j.method = method;
+            j.stop_sample = r.cursor;
+            j.pot = pot_of[k];
+            j.rtol = std::exp(r.req.rtol);
+            j.atol = std::exp(r.req.atol);
+            j.dt_one_init = r.req.dt_one_init;
+            begin += j.n;
+            jobs.push_back(j);
+        }
+        rc = ragged_launch(S->session, jobs.data(), (int32_t)jobs.size(), method, pots, t64, t32,
+                           lo, st, S->scratch[method]);
+        if (rc != DRIFT_OK) return rc;
+    }
+    for (auto& r : active)
+        DRIFT_HIP_TRY(hipMemcpyAsync(&r->host_done, r->d_done, sizeof(unsigned int),
+                                     hipMemcpyDeviceToHost, st));
+    DRIFT_HIP_TRY(hipStreamSynchronize(st));
+    for (auto& r : active) r->info.chunks++;
+    return DRIFT_OK;
+}
+
+void worker_main(drift_server* S) {
+    (void)hipSetDevice(S->session->device);
+    hipStream_t st = S->session->stream;
+    std::vector<std::shared_ptr<Request>> active;
+    for (;;) {
+        std::vector<std::shared_ptr<Request>> admitted;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (active.empty())
+                S->cv_work.wait(lk, [&] { return S->stopping || !S->pending.empty(); });
+            if (S->stopping && S->pending.empty() && active.empty()) break;
+            while (!S->pending.empty()) {
+                auto& r = S->pending.front();
+                const bool alone = active.empty() && admitted.empty();
+                if (!alone && S->used_bytes + r->bytes > S->opt.hbm_budget_bytes) break;
+                S->used_bytes += r->bytes;
+                admitted.push_back(r);
+                S->pending.pop_front();
+            }
+        }
+        for (auto& r : admitted) {
+            int rc = admit(S, *r, st);
+            if (rc != DRIFT_OK) {
+                complete(S, *r, DRIFT_REQUEST_FAILED, rc);
+            } else if (r->req.n == 0) {
+                complete(S, *r, DRIFT_REQUEST_DONE, DRIFT_OK);
+            } else {
+                active.push_back(r);
+            }
+        }
+        if (active.empty()) continue;
+        int rc = run_chunk(S, active, st);
+        std::vector<std::shared_ptr<Request>> still;
+        for (auto& r : active) {
+            if (rc != DRIFT_OK) {
+                complete(S, *r, DRIFT_REQUEST_FAILED, rc);
+            } else if (r->host_done >= r->req.n) {
+                int status = DRIFT_OK;
+                int frc = finish(S, *r, st, status);
+                complete(S, *r, frc == DRIFT_OK ? DRIFT_REQUEST_DONE : DRIFT_REQUEST_FAILED,
+                         frc == DRIFT_OK ? status : frc);
+            } else {
+                still.push_back(r);
+            }
+        }
+        active.swap(still);
+    }
+}
+
+}  // namespace
+}  // namespace drift
+
+using namespace drift;
+
+extern "C" {
+
+int drift_server_create(int device, const drift_server_options* options, drift_server** out) {
+    if (!out) return fail(DRIFT_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    drift_server_options opt{};
+    if (options) opt = *options;
+    if (opt.precision != DRIFT_PRECISION_FP64 && opt.precision != DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "unknown precision mode");
+    if (opt.chunk_samples < 0) return fail(DRIFT_ERR_INVALID_ARGUMENT, "chunk_samples < 0");
+    if (opt.block_size && (opt.block_size < 64 || opt.block_size > 256 || opt.block_size % 64))
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
+    drift_session* sess = nullptr;
+    int rc = drift_session_create(device, &sess);
+    if (rc != DRIFT_OK) return rc;
+    if (opt.hbm_budget_bytes == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            drift_session_destroy(sess);
+            return fail(DRIFT_ERR_DEVICE, "hipMemGetInfo failed");
+        }
+        opt.hbm_budget_bytes = free_b / 2;
+    }
+    drift_server* S = new drift_server();
+    S->session = sess;
+    S->opt = opt;
+    S->t0 = Clock::now();
+    S->worker = std::thread(worker_main, S);
+    *out = S;
+    return DRIFT_OK;
+}
+
+int drift_server_submit(drift_server* S, const drift_request* request, uint64_t* ticket) {
+    if (!S || !request || !ticket) return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL argument");
+    const drift_request& q = *request;
+    if (q.reserved != 0) return fail(DRIFT_ERR_INVALID_ARGUMENT, "drift_request.reserved must be 0");
+    if (q.method != DRIFT_METHOD_DOPR54 && q.method != DRIFT_METHOD_DOP853)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "unknown method");
+    if (q.nt < 2 || q.nt > (size_t)INT32_MAX)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "nt must be in [2, 2^31)");
+    if (!q.times || (q.n > 0 && (!q.state0 || !q.state_out)))
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (!std::isfinite(q.rtol) || !std::isfinite(q.atol))
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "rtol and atol (natural logs) must be finite");
+    if (S->opt.precision == DRIFT_PRECISION_MIXED && std::exp(q.rtol) < 1e-7 * (1 - 1e-9))
+        return fail(DRIFT_ERR_UNSUPPORTED, "mixed precision requires rtol >= 1e-7");
+    int rc = validate_potential(q.potential);
+    if (rc != DRIFT_OK) return rc;
+    auto r = std::make_shared<Request>();
+    r->req = q;
+    r->state0.assign(q.state0 ? q.state0 : (const double*)nullptr,
+                     q.state0 ? q.state0 + q.n * DIM : (const double*)nullptr);
+    r->times.assign(q.times, q.times + q.nt);
+    r->bytes = footprint(q);
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stopping) return fail(DRIFT_ERR_INVALID_ARGUMENT, "server is stopping");
+    r->ticket = S->next_ticket++;
+    r->info.state = DRIFT_REQUEST_QUEUED;
+    r->info.submit_s = now_s(S);
+    S->requests[r->ticket] = r;
+    S->pending.push_back(r);
+    *ticket = r->ticket;
+    S->cv_work.notify_one();
+    return DRIFT_OK;
+}
+
+int drift_server_query(drift_server* S, uint64_t ticket, drift_request_info* info) {
+    if (!S || !info) return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->requests.find(ticket);
+    if (it == S->requests.end()) return fail(DRIFT_ERR_INVALID_ARGUMENT, "unknown ticket");
+    *info = it->second->info;
+    return DRIFT_OK;
+}
+
+int drift_server_wait(drift_server* S, uint64_t ticket, double timeout_s,
+                      drift_request_info* info) {
+    if (!S) return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->requests.find(ticket);
+    if (it == S->requests.end()) return fail(DRIFT_ERR_INVALID_ARGUMENT, "unknown ticket");
+    auto r = it->second;
+    auto finished = [&] {
+        return r->info.state == DRIFT_REQUEST_DONE || r->info.state == DRIFT_REQUEST_FAILED;
+    };
+    if (timeout_s < 0)
+        S->cv_done.wait(lk, finished);
+    else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), finished)) {
+        if (info) *info = r->info;
+        return fail(DRIFT_ERR_TIMEOUT, "request not finished within the timeout");
+    }
+    if (info) *info = r->info;
+    if (r->info.status != DRIFT_OK)
+        return fail(r->info.status, "request " + std::to_string(ticket) + " failed with status " +
+                                        std::to_string(r->info.status));
+    return DRIFT_OK;
+}
+
+int drift_server_destroy(drift_server* S) {
+    if (!S) return DRIFT_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stopping = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    for (auto& sc : S->scratch) {
+        if (sc.desc.ptr) (void)hipFree(sc.desc.ptr);
+        if (sc.queue) (void)hipFree(sc.queue);
+    }
+    drift_session_destroy(S->session);
+    delete S;
+    return DRIFT_OK;
+}
+
+}  // extern "C"

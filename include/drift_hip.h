@@ -48,7 +48,8 @@ typedef enum drift_status {
     DRIFT_ERR_DEVICE = 3,           /* DispatchError::Cuda analogue (HIP runtime error) */
     DRIFT_ERR_INTEGRATION = 4,      /* an orbit returned a non-zero DOPR54 error code
                                        (DispatchError::CpuIntegration analogue) */
-    DRIFT_ERR_OUT_OF_MEMORY = 5
+    DRIFT_ERR_OUT_OF_MEMORY = 5,
+    DRIFT_ERR_TIMEOUT = 6            /* drift_server_wait: not finished in time */
 } drift_status;
 
 typedef enum drift_method {
@@ -157,6 +158,91 @@ int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
                               double rtol, double atol, const drift_potential* potential,
                               drift_orbit_stats* d_stats, const drift_launch_options* options,
                               void* stream);
+
+/* ---- ragged batches -------------------------------------------------------- */
+
+/* One job of a ragged batch: its own orbits, method, potential, time grid and
+ * tolerances (device pointers). */
+typedef struct drift_job {
+    int32_t method;            /* drift_method */
+    int32_t potential;         /* index into the potentials array */
+    size_t n;                  /* orbits (0 allowed) */
+    size_t nt;                 /* output times, >= 2 */
+    double rtol, atol;         /* natural logarithms */
+    double dt_one_init;        /* DOPR54 initial sub-step; -9999.99 = RK4 estimate */
+    double span_hint;          /* |t[nt-1] - t[0]| for scheduling (0 = unknown) */
+    const double* d_state0;    /* (n, 6) */
+    const double* d_times;     /* (nt,) */
+    double* d_state_out;       /* (nt, n, 6) */
+    drift_orbit_stats* d_stats;/* (n,) or NULL */
+} drift_job;
+
+/* Integrates every job to completion: one persistent launch per method, lanes
+ * refilled from a work queue across all jobs of that method (jobs grouped by
+ * potential kind, costliest first).  options->schedule is ignored.  Returns
+ * after the work is complete (it synchronises `stream`). */
+int drift_ragged_device(drift_session* session, const drift_job* jobs, size_t njobs,
+                        const drift_potential* potentials, size_t npotentials,
+                        const drift_launch_options* options, void* stream);
+
+/* ---- serving loop (continuous batching) ---------------------------------- */
+
+/* A request: one job submitted from host memory.  No counterpart in the
+ * reference; it wraps drift_ragged_device's job with host buffers. */
+typedef struct drift_request {
+    int32_t method;               /* drift_method */
+    int32_t reserved;             /* must be 0 */
+    drift_potential potential;
+    size_t n, nt;
+    double rtol, atol;            /* natural logarithms */
+    double dt_one_init;           /* DOPR54; -9999.99 = RK4 estimate */
+    const double* state0;         /* host (n, 6), copied at submission */
+    const double* times;          /* host (nt,), copied at submission */
+    double* state_out;            /* host (nt, n, 6); must stay valid until completion */
+    drift_orbit_stats* stats;     /* host (n,) or NULL */
+} drift_request;
+
+typedef struct drift_server_options {
+    uint64_t hbm_budget_bytes;    /* device bytes for admitted requests; 0 = half the free HBM */
+    int32_t chunk_samples;        /* output samples per orbit per chunk launch; 0 = 8 */
+    int32_t precision;            /* drift_precision for every request */
+    int32_t block_size;           /* 0 = 256 */
+    uint32_t max_attempts;        /* per-orbit attempt limit; 0 = default */
+} drift_server_options;
+
+typedef enum drift_request_state {
+    DRIFT_REQUEST_QUEUED = 0,
+    DRIFT_REQUEST_RUNNING = 1,
+    DRIFT_REQUEST_DONE = 2,
+    DRIFT_REQUEST_FAILED = 3
+} drift_request_state;
+
+typedef struct drift_request_info {
+    int32_t state;                /* drift_request_state */
+    int32_t status;               /* drift_status of a finished request */
+    double submit_s, admit_s, complete_s;  /* seconds since server creation */
+    uint64_t accepted_steps;      /* summed over the request's orbits */
+    uint64_t chunks;              /* chunk launches the request took part in */
+} drift_request_info;
+
+typedef struct drift_server drift_server;
+
+/* Starts a serving loop on `device` (its own session, stream and thread).
+ * Requests are admitted first-come first-served while their device footprint
+ * (inputs, (nt, n, 6) output, resident integrator state) fits the HBM budget;
+ * every chunk launch advances all admitted requests by up to chunk_samples
+ * output samples in one persistent ragged launch per method, their orbit
+ * state resident in HBM between chunks.  A request larger than the budget is
+ * admitted alone. */
+int drift_server_create(int device, const drift_server_options* options, drift_server** out);
+int drift_server_submit(drift_server* server, const drift_request* request, uint64_t* ticket);
+int drift_server_query(drift_server* server, uint64_t ticket, drift_request_info* info);
+/* Blocks until the request finishes or timeout_s elapses (timeout_s < 0:
+ * forever).  Returns the request's drift_status, or DRIFT_ERR_TIMEOUT. */
+int drift_server_wait(drift_server* server, uint64_t ticket, double timeout_s,
+                      drift_request_info* info);
+/* Finishes every submitted request, stops the loop and frees the server. */
+int drift_server_destroy(drift_server* server);
 
 /* ---- force evaluation (test and diagnostics entry) ----------------------- */
 

@@ -1,0 +1,108 @@
+"""Ragged batches: jobs of different particle counts, potentials, methods,
+time grids and tolerances in one persistent launch per method.
+
+Each job's result is compared with the oracle on the same inputs (1e-8
+relative, the parity bar) and with the same job run alone through the plain
+kernel.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from astrodrift_amd import _native, workloads
+
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+POTS = [_native.PotentialSpec.kepler(1.0), _native.PotentialSpec.plummer(1.0, 0.1),
+        _native.PotentialSpec.milky_way()]
+
+
+def disk_ics(spec, n, rng):
+    """Near-circular disk orbits in `spec` (circular speed from the oracle force)."""
+    R = rng.uniform(0.3, 2.0, n)
+    phi = rng.uniform(0, 2 * math.pi, n)
+    z = rng.normal(0, 0.03, n)
+    vc = np.array([math.sqrt(-r * oracle_lib.force(spec, r, 0.0, 0.0)[0]) for r in R])
+    vphi = vc * (1 + rng.normal(0, 0.05, n))
+    vR, vz = rng.normal(0, 0.05, n), rng.normal(0, 0.05, n)
+    c, s = np.cos(phi), np.sin(phi)
+    return np.ascontiguousarray(np.stack([R * c, R * s, z, vR * c - vphi * s, vR * s + vphi * c,
+                                          vz], 1))
+
+
+def make_jobs(count, seed, span=(5.0, 20.0)):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(count):
+        n = int(np.exp(rng.uniform(math.log(16), math.log(512))))
+        pot = int(rng.integers(0, 3))
+        method = int(rng.integers(0, 2))
+        t_end = rng.uniform(*span)
+        nt = int(rng.integers(11, 102))
+        tol = math.exp(rng.uniform(math.log(1e-12), math.log(1e-8)))
+        jobs.append(dict(n=n, pot=pot, method=method, times=workloads.grid_times(0.0, t_end, nt),
+                         tol=tol, y0=disk_ics(POTS[pot], n, rng)))
+    return jobs
+
+
+def run_ragged(jobs, torch):
+    dev = torch.device("cuda:0")
+    keep, descs = [], []
+    for j in jobs:
+        s0 = torch.from_numpy(j["y0"]).to(dev)
+        tt = torch.from_numpy(j["times"]).to(dev)
+        out = torch.full((len(j["times"]), j["n"], 6), float("nan"), dtype=torch.float64,
+                         device=dev)
+        st = torch.zeros((j["n"], 4), dtype=torch.int32, device=dev)
+        keep.append((s0, tt, out, st))
+        lt = math.log(j["tol"])
+        descs.append(_native.DeviceJob(j["method"], j["pot"], j["n"], len(j["times"]), lt, lt,
+                                       s0.data_ptr(), tt.data_ptr(), out.data_ptr(),
+                                       st.data_ptr(), span_hint=float(j["times"][-1])))
+    _native.ragged_device(descs, POTS)
+    torch.cuda.synchronize()
+    return [(o.cpu().numpy(), s.cpu().numpy()) for (_, _, o, s) in keep]
+
+
+def rel(a, b):
+    dp = np.linalg.norm(a[..., :3] - b[..., :3], axis=-1) / np.linalg.norm(b[..., :3], axis=-1)
+    dv = np.linalg.norm(a[..., 3:] - b[..., 3:], axis=-1) / np.linalg.norm(b[..., 3:], axis=-1)
+    return max(dp.max(), dv.max())
+
+
+def test_ragged_batch_matches_oracle_and_plain_kernel(gpu):
+    torch = pytest.importorskip("torch")
+    jobs = make_jobs(24, seed=21)
+    results = run_ragged(jobs, torch)
+    worst_oracle = worst_plain = 0.0
+    for j, (out, st) in zip(jobs, results):
+        lt = math.log(j["tol"])
+        ref, rst = oracle_lib.batch(j["method"], j["y0"], j["times"], lt, lt, POTS[j["pot"]],
+                                    nthreads=8)
+        plain, pst, _ = _native.launch_host(j["method"], j["y0"], j["times"], lt, lt, POTS[j["pot"]])
+        assert not np.isnan(out).any()
+        np.testing.assert_array_equal(out[0], j["y0"])
+        worst_oracle = max(worst_oracle, rel(out, ref))
+        worst_plain = max(worst_plain, rel(out, plain))
+        assert abs(int(st[:, 0].sum()) - int(rst["accepted"].sum())) <= 1e-3 * rst["accepted"].sum()
+        assert np.array_equal(st[:, 0], pst["accepted"].astype(np.int32))
+    print(f"ragged: worst rel vs oracle {worst_oracle:.3e}, vs plain kernel {worst_plain:.3e}")
+    assert worst_oracle < 1e-8
+    assert worst_plain < 1e-10
+
+
+def test_ragged_empty_and_single_jobs(gpu):
+    torch = pytest.importorskip("torch")
+    jobs = make_jobs(1, seed=5)
+    (out, st), = run_ragged(jobs, torch)
+    lt = math.log(jobs[0]["tol"])
+    ref, _ = oracle_lib.batch(jobs[0]["method"], jobs[0]["y0"], jobs[0]["times"], lt, lt,
+                              POTS[jobs[0]["pot"]])
+    assert rel(out, ref) < 1e-8
+    _native.ragged_device([], POTS)  # no jobs: a no-op

@@ -1,0 +1,106 @@
+"""The continuous-batching server: every result is the same alone and in the
+mix (bit for bit: chunking saves and restores the complete integrator state,
+and orbits never interact), agrees with the oracle, and requests submitted
+while others run are admitted and overlap with them."""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import pytest
+
+from astrodrift_amd import _native, serving, workloads
+
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+MW = _native.PotentialSpec.milky_way()
+
+
+def stream_ics(n, seed):
+    """A tidal-stream-like request: Plummer-sphere offsets (scale 0.002,
+    dispersion 0.005) around an eccentric Milky-Way progenitor orbit."""
+    rng = np.random.default_rng(seed)
+    R = rng.uniform(0.5, 2.0)
+    vc = workloads.mw_vcirc(MW, np.array([R]))[0]
+    prog = np.array([R, 0.0, 0.0, 0.0, rng.uniform(0.7, 1.0) * vc, 0.0])
+    # Plummer positions: r = b / sqrt(u^(-2/3) - 1)
+    u = rng.uniform(1e-6, 0.999, n)
+    r = 0.002 / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    y0 = np.tile(prog, (n, 1))
+    y0[:, :3] += np.minimum(r, 0.05)[:, None] * d
+    y0[:, 3:] += rng.normal(0.0, 0.005, (n, 3))
+    return y0
+
+
+def test_alone_equals_mixed_and_matches_oracle(gpu):
+    specs = [(0, 96, 21, 8.0, 1e-10, None), (1, 160, 31, 12.0, 1e-11, MW),
+             (1, 64, 41, 5.0, 1e-9, _native.PotentialSpec.plummer(1.0, 0.1)),
+             (0, 200, 11, 6.0, 1e-10, MW)]
+    reqs = []
+    for k, (method, n, nt, span, tol, pot) in enumerate(specs):
+        y0 = stream_ics(n, seed=100 + k) if pot is MW else \
+            workloads.kepler_disk(n=n, seed=100 + k).state0
+        reqs.append((method, y0, workloads.grid_times(0.0, span, nt), tol, pot))
+
+    # each request alone
+    alone = []
+    with serving.Server(chunk_samples=4) as srv:
+        for method, y0, t, tol, pot in reqs:
+            r = srv.submit(method, y0, t, tol, tol, pot)
+            srv.wait(r, timeout_s=120)
+            alone.append(r.out.copy())
+    # all together, submitted back to back
+    with serving.Server(chunk_samples=4) as srv:
+        handles = [srv.submit(m, y0, t, tol, tol, pot) for m, y0, t, tol, pot in reqs]
+        infos = [srv.wait(h, timeout_s=120) for h in handles]
+        mixed = [h.out for h in handles]
+        for h, info in zip(handles, infos):
+            assert info.state == serving.DONE and info.chunks >= 1
+            assert info.accepted_steps == int(h.stats["accepted"].sum())
+    for (method, y0, t, tol, pot), a, m in zip(reqs, alone, mixed):
+        assert np.array_equal(a, m)  # bit for bit
+        ref, _ = oracle_lib.batch(method, y0, t, math.log(tol), math.log(tol), pot, nthreads=8)
+        dp = np.linalg.norm(m[..., :3] - ref[..., :3], axis=-1) / np.linalg.norm(ref[..., :3], axis=-1)
+        assert dp.max() < 1e-8, dp.max()
+
+
+def test_chunking_does_not_change_results(gpu):
+    w = workloads.mw_disk(n=300, seed=5, t_end=6.0, steps=61, rtol=1e-11, atol=1e-11)
+    outs = []
+    for chunk in (1, 7, 1000):
+        with serving.Server(chunk_samples=chunk) as srv:
+            r = srv.submit(1, w.state0, w.times, w.rtol, w.atol, w.potential)
+            srv.wait(r, timeout_s=120)
+            outs.append(r.out.copy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+
+
+def test_requests_overlap_and_budget_admission(gpu):
+    """A small HBM budget admits requests one or two at a time; later
+    requests are queued, then admitted as earlier ones finish."""
+    w = workloads.mw_disk(n=2000, seed=9, t_end=5.0, steps=51, rtol=1e-10, atol=1e-10)
+    per = 8 * (6 * 2000 + 51 + 6 * 2000 * 51 + 20 * 2000) + 16 * 2000 + 256
+    with serving.Server(chunk_samples=5, hbm_budget_bytes=int(2.5 * per)) as srv:
+        hs = [srv.submit(1, w.state0, w.times, w.rtol, w.atol, w.potential) for _ in range(6)]
+        infos = [srv.wait(h, timeout_s=120) for h in hs]
+    admits = sorted(i.admit_s for i in infos)
+    completes = sorted(i.complete_s for i in infos)
+    # at most two admitted before the first completion
+    assert sum(a <= completes[0] for a in admits) <= 2
+    for h in hs[1:]:
+        assert np.array_equal(h.out, hs[0].out)
+
+
+def test_dopr54_error_code_reaches_the_request(gpu):
+    plunge = np.array([[1.0, 0.0, 0.0, -3.0, 0.0, 0.0]])
+    with serving.Server() as srv:
+        r = srv.submit(0, plunge, workloads.grid_times(0.0, 5.0, 64), 1e-13, 1e-13)
+        with pytest.raises(_native.DriftNativeError):
+            srv.wait(r, timeout_s=60)
+        assert srv.query(r).status == _native.DRIFT_ERR_INTEGRATION
