@@ -36,8 +36,8 @@ enum : int32_t {
 // index from the binary exponent of r2 and whether its mantissa is >= sqrt(2),
 // TABLE_DEG+1 monomial coefficients per segment in the local variable
 // s = (m - centre) * inv_half in [-1, 1], m = r2 / 2^e in [1, 2).
-constexpr int TABLE_E_MIN = -40;
-constexpr int TABLE_E_MAX = 12;
+constexpr int TABLE_E_MIN = -64;
+constexpr int TABLE_E_MAX = 32;
 constexpr int TABLE_DEG = 15;
 constexpr int TABLE_NCOEF = TABLE_DEG + 1;
 constexpr int TABLE_SEGMENTS = 2 * (TABLE_E_MAX - TABLE_E_MIN);
@@ -103,23 +103,60 @@ DRIFT_HD DRIFT_FI T horner_table(const T* c, T s) {
     return acc;
 }
 
-// Direct (slow-path) evaluation of S(r2) outside the tabulated range.
+// ln x for finite x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(z) = 2 (z + z^3/3 + ...), z = (m - 1)/(m + 1), |z| < 0.172,
+// 12 odd terms (truncation < 1e-19); ln 2 split in two parts.  Small in
+// registers, for the slow path below (the hot path uses the table).
+DRIFT_HD DRIFT_FI double light_log(double x) {
+    int e;
+    double m = std::frexp(x, &e);  // [0.5, 1)
+    if (m < 0.70710678118654752) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double z = (m - 1.0) / (m + 1.0);
+    const double z2 = z * z;
+    double p = 1.0 / 23.0;
+#pragma unroll
+    for (int k = 10; k >= 0; --k) p = p * z2 + 1.0 / (2 * k + 1);
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    return (double)e * ln2_hi + ((double)e * ln2_lo + 2.0 * z * p);
+}
+
+// 2^y: 2^n e^(f ln 2), n = round(y), |f| <= 1/2, Taylor series of e^g to
+// degree 16 (|g| <= 0.35, truncation < 1e-19).
+DRIFT_HD DRIFT_FI double light_exp2(double y) {
+    const double n = std::rint(y);
+    const double g = (y - n) * 0.69314718055994530942;
+    double p = 1.0 / 20922789888000.0;  // 1/16!
+    double fact = 20922789888000.0;
+#pragma unroll
+    for (int k = 15; k >= 0; --k) {
+        fact /= (double)(k + 1);
+        p = p * g + 1.0 / fact;
+    }
+    return std::ldexp(p, (int)n);
+}
+
+// Direct (slow-path) evaluation of S(r2) outside the tabulated range
+// [2^E_MIN, 2^E_MAX), i.e. r < 2.3e-10 or r >= 65536.
 DRIFT_HD DRIFT_FI double spherical_direct(const PotParams& p, double r2) {
-    if (r2 == 0.0) return 0.0;
+    if (!(r2 > 0.0)) return 0.0;
     double ir = rsqrt_d(r2);
     double r = r2 * ir;
     double ir3 = ir * ir * ir;
     double sb, sh;
     if (r2 < 1.0) {
-        // r2 < 2^E_MIN: leading terms of the small-radius expansions.
-        sb = std::pow(r2, p.bulge_exponent) * (p.bulge_small_c0 + p.bulge_small_c1 * r2);
+        // leading terms of the small-radius expansions of both components
+        sb = light_exp2(p.bulge_exponent * light_log(r2) * 1.44269504088896340736) *
+             (p.bulge_small_c0 + p.bulge_small_c1 * r2);
         double ia = 1.0 / p.halo_a;
         sh = p.halo_amp * (0.5 * ia * ia * ir - (2.0 / 3.0) * ia * ia * ia + 0.75 * r * ia * ia * ia * ia);
     } else {
-        // r2 >= 2^E_MAX: the bulge is saturated to its total mass.
+        // the bulge has its total mass; u = r/a_h is large, no cancellation
         sb = p.bulge_mass_total * ir3;
         double u = r / p.halo_a;
-        sh = p.halo_amp * (std::log1p(u) - u / (1.0 + u)) * ir3;
+        sh = p.halo_amp * (light_log(1.0 + u) - u / (1.0 + u)) * ir3;
     }
     return sb + sh;
 }

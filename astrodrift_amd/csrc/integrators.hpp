@@ -14,7 +14,9 @@
 //     that emit at least one sample (k10/k2/k3 are recomputed before reuse, so
 //     the trajectory is unchanged);
 //   * DOP853's err^(1/8) is sqrt(sqrt(sqrt(err))); pow(facold, beta) with
-//     beta = 0 is 1 and is dropped;
+//     beta = 0 is 1 and is dropped; divisions by a value used more than once
+//     (the error scale sk, 1/safe, the step h in the dense output) become
+//     multiplications by its reciprocal;
 //   * DOPR54's log-space error norm s = ln(e^(atol-c) + e^(rtol+max-c)) + c is
 //     the logarithm of sc = e^atol + e^rtol max|y|, so the norm is computed as
 //     mean((yerr/sc)^2), and round(log2(0.85 err^-0.2)) is decided by comparing
@@ -289,18 +291,19 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
     double err = 0.0, err2 = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
-        double sk = J.atol + J.rtol * tmax(std::fabs(y0[i]), std::fabs(k5[i]));
+        const double inv_sk = 1.0 / (J.atol + J.rtol * tmax(std::fabs(y0[i]), std::fabs(k5[i])));
         double erri = k4[i] - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i];
-        double sqr = erri / sk;
+        double sqr = erri * inv_sk;
         err2 += sqr * sqr;
         erri = er1 * k1[i] + er6 * k6[i] + er7 * k7[i] + er8 * k8[i] + er9 * k9[i] +
                er10 * k10[i] + er11 * k2[i] + er12 * k3[i];
-        sqr = erri / sk;
+        sqr = erri * inv_sk;
         err += sqr * sqr;
     }
     double deno = err + 0.01 * err2;
     if (deno <= 0.0) deno = 1.0;
-    err = std::fabs(h) * err * std::sqrt(1.0 / (deno * (double)DIM));
+    // sqrt(1/(deno*6)) as a reciprocal square root
+    err = std::fabs(h) * err * rsqrt_d(deno * (double)DIM);
 
     if (err != err) {
         // The reference would shrink h to its floor and retry forever.
@@ -308,7 +311,8 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
         return true;
     }
     const double fac11 = std::sqrt(std::sqrt(std::sqrt(err)));  // err^(1/8)
-    const double fac = tmax(facc2, tmin(facc1, fac11 / safe));
+    constexpr double inv_safe = 1.0 / safe;
+    const double fac = tmax(facc2, tmin(facc1, fac11 * inv_safe));
     double hnew = h / fac;
 
     if (err <= 1.0) {
@@ -316,67 +320,74 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
         f(k5, k4);  // FSAL derivative at the new point
         const double* times = J.times;
         const int32_t last = J.nt - 1;
-        if (L.ii < last && J.pos_neg * times[L.ii + 1] < J.pos_neg * t_new) {
-            // Dense output (dop853.rs:636-763), only when a sample falls in
-            // (t_step, t_new).
+        // samples in (t_step, t_new): dense output (dop853.rs:636-763) only
+        // when there is at least one
+        int32_t emit_n = 0;
+        while (L.ii + emit_n < last && J.pos_neg * times[L.ii + emit_n + 1] < J.pos_neg * t_new)
+            emit_n++;
+        if (emit_n > 0) {
             L.dense++;
-            double r1[DIM], r2[DIM], r3[DIM], r4[DIM], r5[DIM], r6[DIM], r7[DIM], r8[DIM];
-#pragma unroll
-            for (int i = 0; i < DIM; i++) {
-                r1[i] = y0[i];
-                double ydiff = k5[i] - y0[i];
-                r2[i] = ydiff;
-                double bspl = h * k1[i] - ydiff;
-                r3[i] = bspl;
-                r4[i] = ydiff - h * k4[i] - bspl;
-                r5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
-                        d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
-                r6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
-                        d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
-                r7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
-                        d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
-                r8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
-                        d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
-            }
+            // the three extra stages first, into fresh registers (the
+            // reference overwrites k10, k2, k3, dop853.rs:677-732)
+            double k10n[DIM], k2n[DIM], k3n[DIM];
 #pragma unroll
             for (int i = 0; i < DIM; i++)
                 yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
                                       a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] +
                                       a1413 * k4[i]);
-            f(yy1, k10);
+            f(yy1, k10n);
 #pragma unroll
             for (int i = 0; i < DIM; i++)
                 yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
                                       a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] +
-                                      a1514 * k10[i]);
-            f(yy1, k2);
+                                      a1514 * k10n[i]);
+            f(yy1, k2n);
 #pragma unroll
             for (int i = 0; i < DIM; i++)
                 yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
-                                      a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] +
-                                      a1615 * k2[i]);
-            f(yy1, k3);
+                                      a169 * k9[i] + a1613 * k4[i] + a1614 * k10n[i] +
+                                      a1615 * k2n[i]);
+            f(yy1, k3n);
+            // then, one component at a time, the eight dense coefficients and
+            // that component of every sample in the step (only four
+            // coefficients are ever live)
+            bool finite = true;
+            const double inv_h = 1.0 / h;
 #pragma unroll
             for (int i = 0; i < DIM; i++) {
-                r5[i] = h * (r5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
-                r6[i] = h * (r6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
-                r7[i] = h * (r7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
-                r8[i] = h * (r8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
+                const double r1 = y0[i];
+                const double ydiff = k5[i] - y0[i];
+                const double r2 = ydiff;
+                const double bspl = h * k1[i] - ydiff;
+                const double r3 = bspl;
+                const double r4 = ydiff - h * k4[i] - bspl;
+                const double r5 =
+                    h * (d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
+                         d410 * k10[i] + d411 * k2[i] + d412 * k3[i] + d413 * k4[i] +
+                         d414 * k10n[i] + d415 * k2n[i] + d416 * k3n[i]);
+                const double r6 =
+                    h * (d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
+                         d510 * k10[i] + d511 * k2[i] + d512 * k3[i] + d513 * k4[i] +
+                         d514 * k10n[i] + d515 * k2n[i] + d516 * k3n[i]);
+                const double r7 =
+                    h * (d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
+                         d610 * k10[i] + d611 * k2[i] + d612 * k3[i] + d613 * k4[i] +
+                         d614 * k10n[i] + d615 * k2n[i] + d616 * k3n[i]);
+                const double r8 =
+                    h * (d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
+                         d710 * k10[i] + d711 * k2[i] + d712 * k3[i] + d713 * k4[i] +
+                         d714 * k10n[i] + d715 * k2n[i] + d716 * k3n[i]);
+                for (int32_t j = 1; j <= emit_n; j++) {
+                    const double sv = (times[L.ii + j] - t_step) * inv_h;
+                    const double s1 = 1.0 - sv;
+                    const double v =
+                        r1 + sv * (r2 + s1 * (r3 + sv * (r4 + s1 * (r5 + sv * (r6 + s1 * (r7 + sv * r8))))));
+                    finite = finite && std::isfinite(v);
+                    sink.emit_component(L.ii + j, i, v);
+                }
             }
-            do {
-                const double s = (times[L.ii + 1] - t_step) / h;
-                const double s1 = 1.0 - s;
-                double yy[DIM];
-#pragma unroll
-                for (int i = 0; i < DIM; i++)
-                    yy[i] = r1[i] +
-                            s * (r2[i] +
-                                 s1 * (r3[i] +
-                                       s * (r4[i] + s1 * (r5[i] + s * (r6[i] + s1 * (r7[i] + s * r8[i]))))));
-                L.ii++;
-                if (!finite6(yy)) L.status |= STATUS_NONFINITE;
-                sink.emit(L.ii, yy);
-            } while (L.ii < last && J.pos_neg * times[L.ii + 1] < J.pos_neg * t_new);
+            if (!finite) L.status |= STATUS_NONFINITE;
+            L.ii += emit_n;
         }
 #pragma unroll
         for (int i = 0; i < DIM; i++) {
@@ -390,7 +401,7 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
         L.t_cur = t_new;
     } else {
         L.rejected++;
-        hnew = h / tmin(facc1, fac11 / safe);
+        hnew = h / tmin(facc1, fac11 * inv_safe);
         L.reject = 1;
         // t_current = t_old; t_old = t_old_older (dop853.rs:777-778)
         L.t_cur = t_step;
@@ -627,11 +638,11 @@ DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F
 
     // error norm: mean((yerr / (atol + rtol max|yo|))^2)
     const double maxabs = max_abs6(yo);
-    const double sc = J.atol + J.rtol * maxabs;
+    const double inv_sc = 1.0 / (J.atol + J.rtol * maxabs);
     double e = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
-        double q = yerr[i] / sc;
+        double q = yerr[i] * inv_sc;
         e += q * q;
     }
     e = e / (double)DIM;

@@ -173,6 +173,9 @@ struct OutSinkR {
         p[1] = make_double2(y[2], y[3]);
         p[2] = make_double2(y[4], y[5]);
     }
+    __device__ __forceinline__ void emit_component(int32_t s, int c, double v) const {
+        out[((int64_t)s * n + idx) * DIM + c] = v;
+    }
 };
 
 __device__ __forceinline__ bool claim_item(bool need, unsigned long long* queue,

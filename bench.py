@@ -38,7 +38,7 @@ def parse_args():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=1 << 20, help="orbits (default 1,048,576)")
-    p.add_argument("--schedules", default="plain",
+    p.add_argument("--schedules", default="plain,persistent",
                    help="comma list of schedules to time: plain,persistent")
     p.add_argument("--cpu-stride", type=int, default=512,
                    help="CPU baseline integrates every k-th orbit")
@@ -101,7 +101,8 @@ def main() -> int:
     out = torch.empty((w.nt, w.n, 6), dtype=torch.float64, device=dev)
     stats = torch.zeros((w.n, 4), dtype=torch.int32, device=dev)
     tol_r, tol_a = math.log(w.rtol), math.log(w.atol)
-    stream = torch.cuda.current_stream()
+    # a dedicated stream: the kernels and the timing events share it
+    stream = torch.cuda.Stream()
 
     sched_ids = {"plain": _native.SCHEDULE_PLAIN, "persistent": _native.SCHEDULE_PERSISTENT}
     results = {}
