@@ -320,74 +320,68 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
         f(k5, k4);  // FSAL derivative at the new point
         const double* times = J.times;
         const int32_t last = J.nt - 1;
-        // samples in (t_step, t_new): dense output (dop853.rs:636-763) only
-        // when there is at least one
-        int32_t emit_n = 0;
-        while (L.ii + emit_n < last && J.pos_neg * times[L.ii + emit_n + 1] < J.pos_neg * t_new)
-            emit_n++;
-        if (emit_n > 0) {
+        if (L.ii < last && J.pos_neg * times[L.ii + 1] < J.pos_neg * t_new) {
+            // Dense output (dop853.rs:636-763), only when a sample falls in
+            // (t_step, t_new).
             L.dense++;
-            // the three extra stages first, into fresh registers (the
-            // reference overwrites k10, k2, k3, dop853.rs:677-732)
-            double k10n[DIM], k2n[DIM], k3n[DIM];
+            double r1[DIM], r2[DIM], r3[DIM], r4[DIM], r5[DIM], r6[DIM], r7[DIM], r8[DIM];
+#pragma unroll
+            for (int i = 0; i < DIM; i++) {
+                r1[i] = y0[i];
+                double ydiff = k5[i] - y0[i];
+                r2[i] = ydiff;
+                double bspl = h * k1[i] - ydiff;
+                r3[i] = bspl;
+                r4[i] = ydiff - h * k4[i] - bspl;
+                r5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
+                        d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
+                r6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
+                        d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
+                r7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
+                        d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
+                r8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
+                        d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
+            }
 #pragma unroll
             for (int i = 0; i < DIM; i++)
                 yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
                                       a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] +
                                       a1413 * k4[i]);
-            f(yy1, k10n);
+            f(yy1, k10);
 #pragma unroll
             for (int i = 0; i < DIM; i++)
                 yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
                                       a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] +
-                                      a1514 * k10n[i]);
-            f(yy1, k2n);
+                                      a1514 * k10[i]);
+            f(yy1, k2);
 #pragma unroll
             for (int i = 0; i < DIM; i++)
                 yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
-                                      a169 * k9[i] + a1613 * k4[i] + a1614 * k10n[i] +
-                                      a1615 * k2n[i]);
-            f(yy1, k3n);
-            // then, one component at a time, the eight dense coefficients and
-            // that component of every sample in the step (only four
-            // coefficients are ever live)
-            bool finite = true;
-            const double inv_h = 1.0 / h;
+                                      a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] +
+                                      a1615 * k2[i]);
+            f(yy1, k3);
 #pragma unroll
             for (int i = 0; i < DIM; i++) {
-                const double r1 = y0[i];
-                const double ydiff = k5[i] - y0[i];
-                const double r2 = ydiff;
-                const double bspl = h * k1[i] - ydiff;
-                const double r3 = bspl;
-                const double r4 = ydiff - h * k4[i] - bspl;
-                const double r5 =
-                    h * (d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
-                         d410 * k10[i] + d411 * k2[i] + d412 * k3[i] + d413 * k4[i] +
-                         d414 * k10n[i] + d415 * k2n[i] + d416 * k3n[i]);
-                const double r6 =
-                    h * (d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
-                         d510 * k10[i] + d511 * k2[i] + d512 * k3[i] + d513 * k4[i] +
-                         d514 * k10n[i] + d515 * k2n[i] + d516 * k3n[i]);
-                const double r7 =
-                    h * (d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
-                         d610 * k10[i] + d611 * k2[i] + d612 * k3[i] + d613 * k4[i] +
-                         d614 * k10n[i] + d615 * k2n[i] + d616 * k3n[i]);
-                const double r8 =
-                    h * (d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
-                         d710 * k10[i] + d711 * k2[i] + d712 * k3[i] + d713 * k4[i] +
-                         d714 * k10n[i] + d715 * k2n[i] + d716 * k3n[i]);
-                for (int32_t j = 1; j <= emit_n; j++) {
-                    const double sv = (times[L.ii + j] - t_step) * inv_h;
-                    const double s1 = 1.0 - sv;
-                    const double v =
-                        r1 + sv * (r2 + s1 * (r3 + sv * (r4 + s1 * (r5 + sv * (r6 + s1 * (r7 + sv * r8))))));
-                    finite = finite && std::isfinite(v);
-                    sink.emit_component(L.ii + j, i, v);
-                }
+                r5[i] = h * (r5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
+                r6[i] = h * (r6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
+                r7[i] = h * (r7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
+                r8[i] = h * (r8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
             }
-            if (!finite) L.status |= STATUS_NONFINITE;
-            L.ii += emit_n;
+            const double inv_h = 1.0 / h;
+            do {
+                const double sv = (times[L.ii + 1] - t_step) * inv_h;
+                const double s1 = 1.0 - sv;
+                double yy[DIM];
+#pragma unroll
+                for (int i = 0; i < DIM; i++)
+                    yy[i] = r1[i] +
+                            sv * (r2[i] +
+                                  s1 * (r3[i] +
+                                        sv * (r4[i] + s1 * (r5[i] + sv * (r6[i] + s1 * (r7[i] + sv * r8[i]))))));
+                L.ii++;
+                if (!finite6(yy)) L.status |= STATUS_NONFINITE;
+                sink.emit(L.ii, yy);
+            } while (L.ii < last && J.pos_neg * times[L.ii + 1] < J.pos_neg * t_new);
         }
 #pragma unroll
         for (int i = 0; i < DIM; i++) {

@@ -90,6 +90,8 @@ double now_s(const drift_server* S) {
 
 void complete(drift_server* S, Request& r, int state, int status) {
     free_device(r);
+    std::vector<double>().swap(r.state0);  // the server's input copies
+    std::vector<double>().swap(r.times);
     std::lock_guard<std::mutex> lk(S->mu);
     S->used_bytes -= r.bytes;
     r.info.state = state;

@@ -5,6 +5,10 @@
   vertical speed N(0, 0.02); the velocity is then tilted out of the plane by an
   inclination ~ U[0, 0.3] rad about the radial direction.  DOPR54,
   rtol = atol = 1e-10, ts = (0, 20 pi, 101).
+* ``tidal_stream``: one serving request: N stars placed around a progenitor
+  on an eccentric Milky-Way orbit (R ~ U[0.5, 2], speed 0.7-1.0 of circular,
+  tangential) as a Plummer sphere of scale 0.002 with isotropic velocity
+  dispersion 0.005; 101 outputs over a span ~ U[10, 100].
 * ``mw_disk``: 1,048,576 orbits in the Milky-Way model (bulge alpha = 1.8,
   rc = 0.2375, vc^2 fraction 0.05; Miyamoto-Nagai a = 0.375, b = 0.035, 0.6;
   NFW a = 2.0, 0.35), exponential-disk initial conditions: R ~ R exp(-R/0.3125)
@@ -114,3 +118,25 @@ def mw_disk(n: int = 1 << 20, seed: int = MW_SEED, t_end: float = 50.0, steps: i
         "mw_disk_dop853", "DOP853", np.ascontiguousarray(state),
         grid_times(0.0, t_end, steps), rtol, atol, spec,
     )
+
+
+def tidal_stream(n: int, rng: np.random.Generator, span: float | None = None,
+                 steps: int = 101, rtol: float = 1e-10, atol: float = 1e-10) -> Workload:
+    spec = PotentialSpec.milky_way()
+    R = rng.uniform(0.5, 2.0)
+    phi = rng.uniform(0.0, 2.0 * math.pi)
+    v = rng.uniform(0.7, 1.0) * mw_vcirc(spec, np.array([R]))[0]
+    c, s = math.cos(phi), math.sin(phi)
+    prog = np.array([R * c, R * s, 0.0, -v * s, v * c, 0.0])
+    # Plummer sphere: r = b (u^(-2/3) - 1)^(-1/2), isotropic directions,
+    # truncated at 25 scale radii
+    u = rng.uniform(1e-9, 1.0, n)
+    r = np.minimum(0.002 / np.sqrt(u ** (-2.0 / 3.0) - 1.0), 0.05)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    state = np.tile(prog, (n, 1))
+    state[:, :3] += r[:, None] * d
+    state[:, 3:] += rng.normal(0.0, 0.005, (n, 3))
+    t_end = span if span is not None else rng.uniform(10.0, 100.0)
+    return Workload("tidal_stream", "DOP853", np.ascontiguousarray(state),
+                    grid_times(0.0, t_end, steps), rtol, atol, spec)

@@ -10,8 +10,9 @@ rtol = atol = 1e-12, 1,001 dense outputs over t = 0..50, FP64; the
 Prints one JSON line: accepted particle-steps/s of the chosen schedule, the
 plain (one lane per orbit) figure beside it, the roofline of the integrator
 kernel (algorithmic FP64 FLOP/s from the per-orbit step statistics, over the
-kernel's HIP-event duration) and the CPU oracle timed on a strided sample of
-the same orbits.
+kernel's HIP-event duration), the p50/p99 latency of the serving loop under a
+Poisson stream of tidal-stream requests, and the CPU oracle timed on a strided
+sample of the same orbits.
 """
 
 from __future__ import annotations
@@ -43,6 +44,9 @@ def parse_args():
     p.add_argument("--cpu-stride", type=int, default=512,
                    help="CPU baseline integrates every k-th orbit")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--serving-seconds", type=float, default=8.0,
+                   help="length of the Poisson serving run (0 = skip)")
+    p.add_argument("--serving-rate", type=float, default=8.0, help="requests per second")
     return p.parse_args()
 
 
@@ -177,6 +181,14 @@ def main() -> int:
             "traffic": None,
         },
     }
+    if args.serving_seconds > 0:
+        from astrodrift_amd import loadgen
+
+        del out  # release the 50 GB batch output before serving
+        torch.cuda.empty_cache()
+        srv = loadgen.run_poisson(rate_hz=args.serving_rate, duration_s=args.serving_seconds)
+        line["serving"] = srv
+        print(f"[bench] serving: {srv}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, args.cpu_stride)
     print(json.dumps(line), flush=True)
