@@ -39,6 +39,12 @@ PRECISION_MIXED = 1
 SCHEDULE_PLAIN = 0
 SCHEDULE_PERSISTENT = 1
 
+ORDER_AUTO = 0   # cost order for persistent launches of >= 16384 orbits
+ORDER_INDEX = 1
+ORDER_COST = 2
+
+ABI_VERSION = 2
+
 STATUS_CODE_MASK = 0xFF
 STATUS_NONFINITE = 0x100
 STATUS_STEP_LIMIT = 0x200
@@ -96,6 +102,8 @@ class CLaunchOptions(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("schedule", ctypes.c_int32),
         ("max_attempts", ctypes.c_uint32),
+        ("ordering", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -291,7 +299,7 @@ def load_library() -> ctypes.CDLL:
             except OSError as exc:  # pragma: no cover - environment specific
                 raise NativeLibraryError(f"cannot load {LIB_PATH}: {exc}") from exc
             _declare(lib)
-            if lib.drift_get_abi_version() != 1:
+            if lib.drift_get_abi_version() != ABI_VERSION:
                 raise NativeLibraryError("libdrift_hip.so ABI version mismatch")
             _lib = lib
         return _lib
@@ -411,10 +419,11 @@ def launch_device(
     stream: int = 0,
     dt_one: float = INITIAL_STEP_SENTINEL,
     device: int = 0,
+    ordering: int = ORDER_AUTO,
 ) -> None:
     """Device-resident launch on raw device pointers (e.g. torch data_ptr())."""
     s = session(device)
-    opts = CLaunchOptions(precision, block_size, schedule, max_attempts)
+    opts = CLaunchOptions(precision, block_size, schedule, max_attempts, ordering, 0)
     pot = potential.to_c() if potential is not None else None
     pot_ref = ctypes.byref(pot) if pot is not None else None
     args = (s.handle, ctypes.c_void_p(d_state0), ctypes.c_void_p(d_times), ctypes.c_void_p(d_out),
@@ -460,7 +469,7 @@ def ragged_device(jobs: list[DeviceJob], potentials: list, precision: int = PREC
     arr = (CJob * len(jobs))(*[j.to_c() for j in jobs])
     pots = (CPotential * len(potentials))(
         *[(p.to_c() if p is not None else PotentialSpec.galpy_kepler().to_c()) for p in potentials])
-    opts = CLaunchOptions(precision, block_size, 0, max_attempts)
+    opts = CLaunchOptions(precision, block_size, 0, max_attempts, ORDER_AUTO, 0)
     check(s.lib.drift_ragged_device(s.handle, arr, len(jobs), pots, len(potentials),
                                     ctypes.byref(opts), ctypes.c_void_p(stream)))
 

@@ -38,7 +38,8 @@ def compile_command(out: Path = LIB_PATH) -> list[str]:
         "-std=c++17",
         "-fPIC",
         "-shared",
-        # keep the compiler away from packed FP32 forms
+        # no packed FP32 instructions: some of their forms have produced
+        # silently wrong results on this hardware
         "-Xclang",
         "-target-feature",
         "-Xclang",

@@ -59,7 +59,7 @@ struct PlainArgs {
     PotParams pot;
     const void* table;    // MW: TABLE_DOUBLES coefficients (double or float)
     unsigned long long* queue;  // persistent schedule: next unclaimed orbit slot
-    const uint32_t* order;      // persistent schedule: slot -> orbit (NULL = identity)
+    const uint32_t* order;      // slot -> orbit (NULL = identity)
 };
 
 template <int KIND, class FT>
@@ -118,8 +118,9 @@ template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_plain_kernel(PlainArgs a) {
     __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= a.n) return;
+    const int64_t i = a.order ? (int64_t)a.order[slot] : slot;
     const auto f = ForceFor<KIND, FT>::make(a.pot, table);
     const double t0 = a.times[0];
     const Dop853Job J = make_dop853_job(a.times, a.nt, a.rtol, a.atol, a.max_attempts, t0,
@@ -145,8 +146,9 @@ template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) dopr54_plain_kernel(PlainArgs a) {
     __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= a.n) return;
+    const int64_t i = a.order ? (int64_t)a.order[slot] : slot;
     const auto f = ForceFor<KIND, FT>::make(a.pot, table);
     Dopr54Job J;
     J.times = a.times;
@@ -278,6 +280,80 @@ __global__ void __launch_bounds__(kDefaultBlock) dopr54_persistent_kernel(PlainA
 }
 
 __global__ void reset_queue_kernel(unsigned long long* queue) { *queue = 0ull; }
+
+// ---------------------------------------------------------------------------
+// Cost ordering: a counting sort of the orbits by a bucket of their estimated
+// orbital frequency at t0, highest first.  omega^4 = |a|^2 / r^2, bucketed by
+// half-octaves of omega^4 (1/8 octave of omega).  Orbits of similar step
+// counts then share wavefronts, and the long ones start first.
+// ---------------------------------------------------------------------------
+constexpr int kOrderBins = 1024;
+
+__device__ __forceinline__ uint32_t cost_bin(const double* y, const double* d) {
+    const double r2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+    const double a2 = d[3] * d[3] + d[4] * d[4] + d[5] * d[5];
+    const double q = a2 / r2;
+    // no force, a singular or non-finite start: cheapest bin (such orbits end
+    // at once or are free drifts)
+    if (!(q > 0.0) || !(q < INFINITY)) return kOrderBins - 1;
+    int e;
+    const double m = std::frexp(q, &e);  // q = m 2^e, m in [0.5, 1)
+    const int level = 2 * e + (m >= 0.70710678118654752 ? 1 : 0);
+    const int bin = kOrderBins / 2 - 1 - level;
+    return (uint32_t)std::min(std::max(bin, 0), kOrderBins - 1);
+}
+
+__global__ void order_reset_kernel(uint32_t* bins) {
+    for (int k = threadIdx.x; k < kOrderBins; k += blockDim.x) bins[k] = 0u;
+}
+
+template <int KIND, class FT>
+__global__ void __launch_bounds__(kDefaultBlock) order_key_kernel(PlainArgs a, uint16_t* keys,
+                                                                   uint32_t* bins) {
+    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    __shared__ uint32_t local[kOrderBins];
+    for (int k = threadIdx.x; k < kOrderBins; k += blockDim.x) local[k] = 0u;
+    const FT* table = stage_table<KIND, FT>(a.table, lds);  // ends with a barrier
+    if constexpr (KIND != POT_MW) __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n) {
+        const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+        double y[DIM], d[DIM];
+        load_state(a.state0, i, y);
+        f(y, d);
+        const uint32_t b = cost_bin(y, d);
+        keys[i] = (uint16_t)b;
+        atomicAdd(&local[b], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kOrderBins; k += blockDim.x)
+        if (local[k]) atomicAdd(&bins[k], local[k]);
+}
+
+// Exclusive prefix sum of the bin counts, in place; one block of kOrderBins
+// threads.
+__global__ void __launch_bounds__(kOrderBins) order_scan_kernel(uint32_t* bins) {
+    __shared__ uint32_t s[kOrderBins];
+    const int t = threadIdx.x;
+    const uint32_t own = bins[t];
+    s[t] = own;
+    __syncthreads();
+    for (int off = 1; off < kOrderBins; off <<= 1) {
+        const uint32_t v = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    bins[t] = s[t] - own;
+}
+
+__global__ void __launch_bounds__(kDefaultBlock) order_scatter_kernel(const uint16_t* keys, int64_t n,
+                                                                       uint32_t* bins, uint32_t* order) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t slot = atomicAdd(&bins[keys[i]], 1u);
+    order[slot] = (uint32_t)i;
+}
 
 struct ForceArgs {
     const double* pos;
@@ -446,6 +522,55 @@ PlainKernel kernel_for(int method, int kind, int precision, int schedule) {
     }
 }
 
+using KeyKernel = void (*)(PlainArgs, uint16_t*, uint32_t*);
+
+template <int KIND>
+KeyKernel pick_key(int precision) {
+    return precision == DRIFT_PRECISION_MIXED ? order_key_kernel<KIND, float>
+                                              : order_key_kernel<KIND, double>;
+}
+
+KeyKernel key_kernel_for(int kind, int precision) {
+    switch (kind) {
+        case POT_GALPY_KEPLER: return pick_key<POT_GALPY_KEPLER>(precision);
+        case POT_KEPLER: return pick_key<POT_KEPLER>(precision);
+        case POT_PLUMMER: return pick_key<POT_PLUMMER>(precision);
+        default: return pick_key<POT_MW>(precision);
+    }
+}
+
+// Orbit counts from which DRIFT_ORDER_AUTO sorts a persistent launch: below
+// it the machine is not full and the counting sort would not pay for itself.
+constexpr size_t kAutoOrderMinOrbits = 16384;
+
+// Builds s->order_slots (slot -> orbit, most expensive bin first) on stream
+// st.  The launch that reads it records s->order_done after its integration
+// kernel; the next build waits for that event, so launches on different
+// streams never overwrite a map still in use.
+int build_cost_order(drift_session* s, const PlainArgs& a, int precision, hipStream_t st) {
+    const size_t n = (size_t)a.n;
+    int rc;
+    if ((rc = ensure_buffer(s->order_keys, n * sizeof(uint16_t))) != DRIFT_OK) return rc;
+    if ((rc = ensure_buffer(s->order_slots, n * sizeof(uint32_t))) != DRIFT_OK) return rc;
+    if ((rc = ensure_buffer(s->order_bins, kOrderBins * sizeof(uint32_t))) != DRIFT_OK) return rc;
+    if (!s->order_done)
+        DRIFT_HIP_TRY(hipEventCreateWithFlags(&s->order_done, hipEventDisableTiming));
+    else
+        DRIFT_HIP_TRY(hipStreamWaitEvent(st, s->order_done, 0));
+    uint16_t* keys = static_cast<uint16_t*>(s->order_keys.ptr);
+    uint32_t* bins = static_cast<uint32_t*>(s->order_bins.ptr);
+    uint32_t* slots = static_cast<uint32_t*>(s->order_slots.ptr);
+    const unsigned blocks = (unsigned)((n + kDefaultBlock - 1) / kDefaultBlock);
+    hipLaunchKernelGGL(order_reset_kernel, dim3(1), dim3(256), 0, st, bins);
+    hipLaunchKernelGGL(key_kernel_for(a.pot.kind, precision), dim3(blocks), dim3(kDefaultBlock), 0,
+                       st, a, keys, bins);
+    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(kOrderBins), 0, st, bins);
+    hipLaunchKernelGGL(order_scatter_kernel, dim3(blocks), dim3(kDefaultBlock), 0, st,
+                       (const uint16_t*)keys, a.n, bins, slots);
+    DRIFT_HIP_TRY(hipGetLastError());
+    return DRIFT_OK;
+}
+
 constexpr unsigned kQueueRing = 64;
 
 constexpr uint32_t kDefaultMaxAttempts = 1u << 22;
@@ -475,6 +600,10 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
     if (opt.schedule != 0 && opt.schedule != 1)
         return fail(DRIFT_ERR_UNSUPPORTED, "schedule must be 0 (plain grid) or 1 (persistent)");
+    if (opt.ordering < DRIFT_ORDER_AUTO || opt.ordering > DRIFT_ORDER_COST)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "ordering must be AUTO, INDEX or COST");
+    if (opt.reserved != 0)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "drift_launch_options.reserved must be 0");
 
     PlainArgs a{};
     a.state0 = d_state0;
@@ -513,8 +642,18 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         DRIFT_HIP_TRY(hipGetLastError());
     }
     if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");
+    const bool cost_order =
+        opt.ordering == DRIFT_ORDER_COST ||
+        (opt.ordering == DRIFT_ORDER_AUTO && opt.schedule == 1 && n >= kAutoOrderMinOrbits);
+    if (cost_order) {
+        if (n > (size_t)UINT32_MAX)
+            return fail(DRIFT_ERR_INVALID_ARGUMENT, "cost ordering supports n < 2^32");
+        if ((rc = build_cost_order(s, a, opt.precision, st)) != DRIFT_OK) return rc;
+        a.order = static_cast<const uint32_t*>(s->order_slots.ptr);
+    }
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, st, a);
     DRIFT_HIP_TRY(hipGetLastError());
+    if (cost_order) DRIFT_HIP_TRY(hipEventRecord(s->order_done, st));
     return DRIFT_OK;
 }
 
@@ -537,9 +676,13 @@ int host_launch(drift_session* s, int method, const double* state0, const double
     DRIFT_HIP_TRY(hipMemcpyAsync(s->state0.ptr, state0, in_bytes, hipMemcpyHostToDevice, s->stream));
     DRIFT_HIP_TRY(hipMemcpyAsync(s->times.ptr, times, nt * sizeof(double), hipMemcpyHostToDevice,
                                  s->stream));
+    // the fastest schedule: persistent lanes, cost-ordered from 16384 orbits
+    drift_launch_options opt{};
+    opt.schedule = 1;
+    opt.ordering = DRIFT_ORDER_AUTO;
     rc = launch_integrator(s, method, (const double*)s->state0.ptr, (const double*)s->times.ptr,
                            (double*)s->out.ptr, n, nt, rtol_ln, atol_ln, dt_one_init, potential,
-                           (drift_orbit_stats*)s->stats.ptr, nullptr, s->stream);
+                           (drift_orbit_stats*)s->stats.ptr, &opt, s->stream);
     if (rc != DRIFT_OK) return rc;
     std::vector<drift_orbit_stats> host_stats;
     drift_orbit_stats* st = stats;
@@ -607,8 +750,10 @@ int drift_session_destroy(drift_session* s) {
         (void)hipFree(t.d_table64);
         (void)hipFree(t.d_table32);
     }
-    for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats})
+    for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats, &s->order_keys,
+                            &s->order_slots, &s->order_bins})
         if (b->ptr) (void)hipFree(b->ptr);
+    if (s->order_done) (void)hipEventDestroy(s->order_done);
     if (s->queues) (void)hipFree(s->queues);
     for (auto& r : s->ragged) {
         if (r.desc.ptr) (void)hipFree(r.desc.ptr);

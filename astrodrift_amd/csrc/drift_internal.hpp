@@ -41,6 +41,10 @@ struct drift_session {
     unsigned queue_next = 0;
     // descriptor buffers of drift_ragged_device, one per method
     drift::RaggedScratch ragged[2];
+    // cost ordering: per-orbit bin keys, the slot -> orbit map and the bin
+    // counters; order_done marks the end of the last launch that read them
+    drift::DeviceBuffer order_keys, order_slots, order_bins;
+    hipEvent_t order_done = nullptr;
 };
 
 #define DRIFT_HIP_TRY(expr)                                                              \

@@ -23,7 +23,7 @@ def main() -> int:
     p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     p.add_argument("--n", type=int, default=1 << 20)
     p.add_argument("--schedule", type=int, default=1, help="0 plain, 1 persistent")
-    p.add_argument("--copy-gib", type=float, default=2.0)
+    p.add_argument("--copy-gib", type=float, default=2.0, help="0 skips the calibration copy")
     args = p.parse_args()
 
     import torch
@@ -33,15 +33,16 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     n_copy = int(args.copy_gib * (1 << 30) // 8)
-    a = torch.ones(n_copy, dtype=torch.float64, device=dev)
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
-    print(f"[profiling] calibration copy: {n_copy * 8} bytes read, {n_copy * 8} bytes written",
-          flush=True)
-    del a, b
-    torch.cuda.empty_cache()
+    if n_copy > 0:
+        a = torch.ones(n_copy, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"[profiling] calibration copy: {n_copy * 8} bytes read, {n_copy * 8} bytes written",
+              flush=True)
+        del a, b
+        torch.cuda.empty_cache()
 
     w = workloads.mw_disk(n=args.n)
     s0 = torch.from_numpy(w.state0).to(dev)

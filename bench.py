@@ -39,8 +39,9 @@ def parse_args():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=1 << 20, help="orbits (default 1,048,576)")
-    p.add_argument("--schedules", default="plain,persistent",
-                   help="comma list of schedules to time: plain,persistent")
+    p.add_argument("--schedules", default="plain,persistent:index,persistent",
+                   help="comma list of schedule[:ordering] to time; schedule plain|persistent, "
+                        "ordering auto|index|cost (auto: cost order for persistent launches)")
     p.add_argument("--cpu-stride", type=int, default=512,
                    help="CPU baseline integrates every k-th orbit")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -109,14 +110,21 @@ def main() -> int:
     stream = torch.cuda.Stream()
 
     sched_ids = {"plain": _native.SCHEDULE_PLAIN, "persistent": _native.SCHEDULE_PERSISTENT}
+    order_ids = {"auto": _native.ORDER_AUTO, "index": _native.ORDER_INDEX,
+                 "cost": _native.ORDER_COST}
     results = {}
     for name in [s.strip() for s in args.schedules.split(",") if s.strip()]:
-        schedule = sched_ids[name]
+        # "schedule" or "schedule:ordering"; the default ordering (auto) sorts
+        # persistent launches by estimated orbit cost and leaves plain ones in
+        # index order
+        sched_name, _, order_name = name.partition(":")
+        schedule = sched_ids[sched_name]
+        ordering = order_ids[order_name or "auto"]
 
         def launch():
             _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(),
                                   out.data_ptr(), w.n, w.nt, tol_r, tol_a, w.potential,
-                                  stats.data_ptr(), schedule=schedule,
+                                  stats.data_ptr(), schedule=schedule, ordering=ordering,
                                   stream=stream.cuda_stream)
 
         for _ in range(args.warmup):

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define DRIFT_ABI_VERSION 1
+#define DRIFT_ABI_VERSION 2
 
 /* Status codes.  They map onto the reference's error taxonomy
  * (DispatchError, src/dispatch/mod.rs:36-79; IntegrationError,
@@ -107,7 +107,22 @@ typedef struct drift_launch_options {
                               The reference has none: a DOP853 orbit whose error
                               estimate never drops below 1 retries forever at the
                               step floor. */
+    int32_t ordering;   /* drift_ordering: the order in which lanes take orbits */
+    int32_t reserved;   /* must be 0 */
 } drift_launch_options;
+
+/* Orbit order of a launch.  Outputs never depend on it (orbits are
+ * independent); it only changes how evenly the lanes of a wavefront are
+ * loaded.  COST sorts the orbits, on the device, by the estimated orbital
+ * frequency sqrt(|a|/r) at t0 (1/8-octave buckets, highest first), so that a
+ * wavefront's lanes carry orbits of similar step counts and the expensive ones
+ * start first.  It uses per-session scratch buffers: launches of one session
+ * that use it are serialised with each other, across streams, by an event. */
+typedef enum drift_ordering {
+    DRIFT_ORDER_AUTO = 0,  /* COST for the persistent schedule with n >= 16384, else INDEX */
+    DRIFT_ORDER_INDEX = 1, /* orbit i is taken in index order */
+    DRIFT_ORDER_COST = 2
+} drift_ordering;
 
 typedef struct drift_session drift_session;
 

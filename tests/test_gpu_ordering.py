@@ -1,0 +1,117 @@
+"""Cost ordering of the orbits (drift_launch_options.ordering).
+
+The order in which lanes take orbits must not change any output: every
+schedule/ordering combination gives bit-identical trajectories and step
+statistics, including for orbits that start at a singular point.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from astrodrift_amd import _native, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(w, method, schedule, ordering, precision=_native.PRECISION_FP64, state0=None):
+    import torch
+
+    dev = torch.device("cuda:0")
+    y0 = w.state0 if state0 is None else state0
+    n = y0.shape[0]
+    s0 = torch.from_numpy(np.ascontiguousarray(y0)).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    out = torch.full((w.nt, n, 6), float("nan"), dtype=torch.float64, device=dev)
+    st = torch.zeros((n, 4), dtype=torch.int32, device=dev)
+    _native.launch_device(method, s0.data_ptr(), tt.data_ptr(), out.data_ptr(), n, w.nt,
+                          math.log(w.rtol), math.log(w.atol), w.potential, st.data_ptr(),
+                          precision=precision, schedule=schedule, ordering=ordering)
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), st.cpu().numpy()
+
+
+def _same(a, b):
+    # bitwise, NaN included
+    return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+COMBOS = [
+    (_native.SCHEDULE_PERSISTENT, _native.ORDER_COST),
+    (_native.SCHEDULE_PERSISTENT, _native.ORDER_AUTO),
+    (_native.SCHEDULE_PLAIN, _native.ORDER_COST),
+    (_native.SCHEDULE_PLAIN, _native.ORDER_INDEX),
+]
+
+
+def test_mw_dop853_every_order_is_bit_identical(gpu):
+    """20,000 Milky-Way orbits (above the AUTO threshold), t = 0..10, 201 outputs."""
+    w = workloads.mw_disk(n=20000, t_end=10.0, steps=201)
+    ref_out, ref_st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_PERSISTENT,
+                           _native.ORDER_INDEX)
+    assert not np.isnan(ref_out).any()
+    for schedule, ordering in COMBOS:
+        out, st = _run(w, _native.METHOD_DOP853, schedule, ordering)
+        print(f"schedule={schedule} ordering={ordering}: accepted={int(st[:, 0].sum())}")
+        assert _same(out, ref_out), (schedule, ordering)
+        assert np.array_equal(st, ref_st), (schedule, ordering)
+
+
+def test_kepler_dopr54_cost_order_is_bit_identical(gpu):
+    w = workloads.kepler_disk(n=20000)
+    ref_out, ref_st = _run(w, _native.METHOD_DOPR54, _native.SCHEDULE_PLAIN, _native.ORDER_INDEX)
+    for schedule, ordering in COMBOS[:3]:
+        out, st = _run(w, _native.METHOD_DOPR54, schedule, ordering)
+        assert _same(out, ref_out), (schedule, ordering)
+        assert np.array_equal(st, ref_st), (schedule, ordering)
+
+
+def test_mixed_precision_cost_order_is_bit_identical(gpu):
+    w = workloads.mw_disk(n=20000, t_end=10.0, steps=101, rtol=1e-7, atol=1e-7)
+    ref_out, ref_st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_PERSISTENT,
+                           _native.ORDER_INDEX, precision=_native.PRECISION_MIXED)
+    out, st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_PERSISTENT, _native.ORDER_COST,
+                   precision=_native.PRECISION_MIXED)
+    assert _same(out, ref_out)
+    assert np.array_equal(st, ref_st)
+
+
+def test_singular_and_free_starts_are_ordered_last_without_changing_results(gpu):
+    """Orbits at the origin of the galpy Kepler force (NaN force) and orbits far
+    out (tiny force) mixed into a normal batch."""
+    w = workloads.kepler_disk(n=4096)
+    w.potential = None  # the reference's unit galpy Kepler force
+    y0 = w.state0.copy()
+    y0[::97, :3] = 0.0         # at the singularity
+    y0[5::101, :3] *= 1e6      # nearly free drifts
+    ref_out, ref_st = _run(w, _native.METHOD_DOPR54, _native.SCHEDULE_PERSISTENT,
+                           _native.ORDER_INDEX, state0=y0)
+    out, st = _run(w, _native.METHOD_DOPR54, _native.SCHEDULE_PERSISTENT, _native.ORDER_COST,
+                   state0=y0)
+    assert _same(out, ref_out)
+    assert np.array_equal(st, ref_st)
+
+
+def test_bad_ordering_options_are_rejected(gpu):
+    import torch
+
+    w = workloads.kepler_disk(n=64)
+    dev = torch.device("cuda:0")
+    s0 = torch.from_numpy(w.state0).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    out = torch.empty((w.nt, w.n, 6), dtype=torch.float64, device=dev)
+    with pytest.raises(_native.DriftNativeError, match="ordering"):
+        _native.launch_device(_native.METHOD_DOPR54, s0.data_ptr(), tt.data_ptr(), out.data_ptr(),
+                              w.n, w.nt, -23.0, -23.0, ordering=3)
+    s = _native.session(0)
+    opts = _native.CLaunchOptions(0, 0, 1, 0, 0, 7)
+    rc = s.lib.drift_galpy_dopr54_device(
+        s.handle, ctypes.c_void_p(s0.data_ptr()), ctypes.c_void_p(tt.data_ptr()),
+        ctypes.c_void_p(out.data_ptr()), w.n, w.nt, -23.0, -23.0, _native.INITIAL_STEP_SENTINEL,
+        None, None, ctypes.byref(opts), None)
+    assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT
+    assert "reserved" in _native.last_error()
