@@ -347,12 +347,39 @@ __global__ void __launch_bounds__(kOrderBins) order_scan_kernel(uint32_t* bins) 
     bins[t] = s[t] - own;
 }
 
-__global__ void __launch_bounds__(kDefaultBlock) order_scatter_kernel(const uint16_t* keys, int64_t n,
-                                                                       uint32_t* bins, uint32_t* order) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t slot = atomicAdd(&bins[keys[i]], 1u);
-    order[slot] = (uint32_t)i;
+// Each block places kScatterItems consecutive orbits: ranks within a bin come
+// from LDS atomics, and one global atomic per (block, non-empty bin) reserves
+// the block's range of that bin, so hot bins see few global atomics.
+constexpr int kScatterThreads = kOrderBins;
+constexpr int kScatterPerThread = 4;
+constexpr int kScatterItems = kScatterThreads * kScatterPerThread;
+
+__global__ void __launch_bounds__(kScatterThreads) order_scatter_kernel(const uint16_t* keys,
+                                                                        int64_t n, uint32_t* bins,
+                                                                        uint32_t* order) {
+    __shared__ uint32_t count[kOrderBins];
+    __shared__ uint32_t base[kOrderBins];
+    const int t = threadIdx.x;
+    count[t] = 0u;
+    __syncthreads();
+    const int64_t first = (int64_t)blockIdx.x * kScatterItems;
+    uint32_t key[kScatterPerThread], rank[kScatterPerThread];
+#pragma unroll
+    for (int k = 0; k < kScatterPerThread; ++k) {
+        const int64_t i = first + (int64_t)k * kScatterThreads + t;
+        if (i < n) {
+            key[k] = keys[i];
+            rank[k] = atomicAdd(&count[key[k]], 1u);
+        }
+    }
+    __syncthreads();
+    if (count[t]) base[t] = atomicAdd(&bins[t], count[t]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScatterPerThread; ++k) {
+        const int64_t i = first + (int64_t)k * kScatterThreads + t;
+        if (i < n) order[base[key[k]] + rank[k]] = (uint32_t)i;
+    }
 }
 
 struct ForceArgs {
@@ -565,7 +592,8 @@ int build_cost_order(drift_session* s, const PlainArgs& a, int precision, hipStr
     hipLaunchKernelGGL(key_kernel_for(a.pot.kind, precision), dim3(blocks), dim3(kDefaultBlock), 0,
                        st, a, keys, bins);
     hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(kOrderBins), 0, st, bins);
-    hipLaunchKernelGGL(order_scatter_kernel, dim3(blocks), dim3(kDefaultBlock), 0, st,
+    const unsigned scatter_blocks = (unsigned)((n + kScatterItems - 1) / kScatterItems);
+    hipLaunchKernelGGL(order_scatter_kernel, dim3(scatter_blocks), dim3(kScatterThreads), 0, st,
                        (const uint16_t*)keys, a.n, bins, slots);
     DRIFT_HIP_TRY(hipGetLastError());
     return DRIFT_OK;

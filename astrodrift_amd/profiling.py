@@ -1,31 +1,46 @@
-"""One profiling pass of the benchmark kernel, for rocprofv3.
+"""Profiling driver and summarizer for rocprofv3 runs of the benchmark kernel.
 
-    rocprofv3 --kernel-trace --stats -- python3 -m astrodrift_amd.profiling
-    rocprofv3 --pmc FETCH_SIZE -- python3 -m astrodrift_amd.profiling
-    rocprofv3 --pmc WRITE_SIZE -- python3 -m astrodrift_amd.profiling
+Run (one pass each; counters in runs of their own):
 
-Launches, in this order on one stream: a calibration copy of a known size
-(torch's copy of a 2 GiB float64 tensor: 2 GiB read, 2 GiB written), then
-one integration of the 1,048,576-orbit Milky-Way DOP853 workload (rtol =
-atol = 1e-12, 1,001 outputs over t = 0..50) with the persistent schedule.
-The copy's counter values give the factor that turns FETCH_SIZE / WRITE_SIZE
-into bytes for the integrator's dispatch.
+    rocprofv3 --kernel-trace --stats --output-format csv -d D/trace -o run -- \\
+        python3 -m astrodrift_amd.profiling
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d D/fetch -o run -- \\
+        python3 -m astrodrift_amd.profiling
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d D/write -o run -- \\
+        python3 -m astrodrift_amd.profiling
+
+The run launches, on one stream: a calibration copy of a known size (torch's
+copy of a 2 GiB float64 tensor: 2 GiB read, 2 GiB written), then one
+integration of the 1,048,576-orbit Milky-Way DOP853 workload (rtol = atol =
+1e-12, 1,001 outputs over t = 0..50) with the persistent schedule and the
+default (cost) orbit ordering.  The copy's counter values give the factor that
+turns FETCH_SIZE / WRITE_SIZE into bytes for the integrator's dispatch.
+
+Summarize (reads the CSV files rocprofv3 wrote, prints markdown):
+
+    python3 -m astrodrift_amd.profiling --summarize D/trace [D/fetch D/write [D/sq ...]]
 """
 
 from __future__ import annotations
 
 import argparse
+import csv
 import math
 import sys
+from pathlib import Path
+
+INTEGRATOR = "dop853_persistent_kernel"
+COPY = "__amd_rocclr_copyBuffer"
+ALGORITHMIC_BYTES_PER_ORBIT_SAMPLE = 48  # one (x, y, z, vx, vy, vz) record
 
 
-def main() -> int:
-    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    p.add_argument("--n", type=int, default=1 << 20)
-    p.add_argument("--schedule", type=int, default=1, help="0 plain, 1 persistent")
-    p.add_argument("--copy-gib", type=float, default=2.0, help="0 skips the calibration copy")
-    args = p.parse_args()
+def algorithmic_bytes(n: int, nt: int) -> int:
+    """Bytes one launch must move: state0 in (48 n), the time grid (8 nt), the
+    trajectories out (48 n nt) and the statistics out (16 n)."""
+    return 48 * n + 8 * nt + ALGORITHMIC_BYTES_PER_ORBIT_SAMPLE * n * nt + 16 * n
 
+
+def run(args) -> int:
     import torch
 
     from . import _native, workloads
@@ -54,10 +69,104 @@ def main() -> int:
                           w.n, w.nt, lt, lt, w.potential, st.data_ptr(), schedule=args.schedule)
     torch.cuda.synchronize()
     acc = int(st[:, 0].sum().item())
-    algorithmic_bytes = 48 * w.n + 8 * w.nt + 48 * w.n * w.nt + 16 * w.n
     print(f"[profiling] integrator: n={w.n} nt={w.nt} accepted={acc} "
-          f"algorithmic bytes={algorithmic_bytes}", flush=True)
+          f"algorithmic bytes={algorithmic_bytes(w.n, w.nt)}", flush=True)
     return 0
+
+
+def _csv_rows(d: Path, suffix: str) -> list[dict]:
+    files = sorted(Path(d).rglob(f"*{suffix}"))
+    if not files:
+        raise FileNotFoundError(f"no *{suffix} under {d}")
+    rows = []
+    for f in files:
+        with open(f, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _counters(d: Path) -> dict[str, dict[str, float]]:
+    """'kernel [dispatch id]' -> counter -> value summed over the dispatch's
+    instances (one per XCD / shader engine)."""
+    per_dispatch: dict[tuple[str, str], dict[str, float]] = {}
+    for r in _csv_rows(d, "counter_collection.csv"):
+        key = (r["Dispatch_Id"], r["Kernel_Name"])
+        c = per_dispatch.setdefault(key, {})
+        c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return {f"{name} [dispatch {did}]": c for (did, name), c in per_dispatch.items()}
+
+
+def _pick(counters: dict, needle: str, counter: str) -> float:
+    vals = [c[counter] for k, c in counters.items() if needle in k and counter in c]
+    if not vals:
+        raise KeyError(f"{counter} of {needle} not found")
+    return max(vals)  # the large dispatch of that name
+
+
+def summarize(dirs: list[str]) -> int:
+    trace = Path(dirs[0])
+    print("| kernel | calls | avg duration (ms) | total (ms) | share |")
+    print("|---|---|---|---|---|")
+    stats = _csv_rows(trace, "kernel_stats.csv")
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:8]:
+        name = r["Name"] if len(r["Name"]) < 90 else r["Name"][:87] + "..."
+        print(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.3f} | "
+              f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.2f}% |")
+    if len(dirs) == 1:
+        return 0
+    fetch, write, *extra = [Path(d) for d in dirs[1:]]
+    fc, wc = _counters(fetch), _counters(write)
+    copy_bytes = 2 * (1 << 30)
+    f_copy = _pick(fc, COPY, "FETCH_SIZE")
+    w_copy = _pick(wc, COPY, "WRITE_SIZE")
+    f_int = _pick(fc, INTEGRATOR, "FETCH_SIZE")
+    w_int = _pick(wc, INTEGRATOR, "WRITE_SIZE")
+    f_factor = copy_bytes / f_copy
+    w_factor = copy_bytes / w_copy
+    fetch_b = f_int * f_factor
+    write_b = w_int * w_factor
+    alg = algorithmic_bytes(1 << 20, 1001)
+    print()
+    print("| quantity | value |")
+    print("|---|---|")
+    print(f"| calibration copy FETCH_SIZE (2 GiB read) | {f_copy:.1f} -> {f_factor:.4f} bytes/unit |")
+    print(f"| calibration copy WRITE_SIZE (2 GiB written) | {w_copy:.1f} -> {w_factor:.4f} bytes/unit |")
+    print(f"| integrator FETCH_SIZE | {f_int:.1f} -> {fetch_b / 1e9:.3f} GB |")
+    print(f"| integrator WRITE_SIZE | {w_int:.1f} -> {write_b / 1e9:.3f} GB |")
+    print(f"| traffic per launch (fetch + write) | {(fetch_b + write_b) / 1e9:.3f} GB |")
+    print(f"| algorithmic bytes per launch | {alg / 1e9:.3f} GB |")
+    print(f"| traffic / algorithmic | {(fetch_b + write_b) / alg:.3f} |")
+    print()
+    print(f"traffic_bytes_per_launch: {int(round(fetch_b + write_b))}")
+    for d in extra:
+        cc = _counters(d)
+        print()
+        print(f"Counters from `{d.name}` (integrator dispatch):")
+        print()
+        print("| counter | value |")
+        print("|---|---|")
+        for k, c in cc.items():
+            if INTEGRATOR in k:
+                for name, v in sorted(c.items()):
+                    print(f"| {name} | {v:.6e} |")
+                break
+    return 0
+
+
+def main() -> int:
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--n", type=int, default=1 << 20)
+    p.add_argument("--schedule", type=int, default=1, help="0 plain, 1 persistent")
+    p.add_argument("--copy-gib", type=float, default=2.0, help="0 skips the calibration copy")
+    p.add_argument("--summarize", nargs="+", metavar="DIR",
+                   help="trace folder, then optionally the fetch and write folders and more "
+                        "counter folders")
+    args = p.parse_args()
+    if args.summarize:
+        if len(args.summarize) == 2:
+            p.error("--summarize takes the trace folder, then the fetch and write folders")
+        return summarize(args.summarize)
+    return run(args)
 
 
 if __name__ == "__main__":

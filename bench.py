@@ -51,6 +51,21 @@ def parse_args():
     return p.parse_args()
 
 
+TRAFFIC_PROFILE = REPO / "profiles" / "dop853_mw_1M_traffic.md"
+
+
+def profiled_traffic(n: int, schedule: str):
+    """HBM bytes per launch of the persistent 1M-orbit integrator, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (a counter run cannot
+    share a process with the timed loop); None for any other configuration."""
+    if n != 1 << 20 or schedule != "persistent" or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(w, stride: int):
     """The oracle (the reference CPU algorithm restated) on orbits 0, stride,
     2*stride, ...: one thread (as the reference engine), then all host cores."""
@@ -186,7 +201,8 @@ def main() -> int:
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": r["tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": profiled_traffic(w.n, chosen),
+            "traffic_source": str(TRAFFIC_PROFILE.relative_to(REPO)),
         },
     }
     if args.serving_seconds > 0:
