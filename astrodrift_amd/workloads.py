@@ -9,6 +9,13 @@
   on an eccentric Milky-Way orbit (R ~ U[0.5, 2], speed 0.7-1.0 of circular,
   tangential) as a Plummer sphere of scale 0.002 with isotropic velocity
   dispersion 0.005; 101 outputs over a span ~ U[10, 100].
+* ``ragged_mix``: a ragged batch of jobs: n_j log-uniform on [16, 65,536],
+  potential uniform over {Kepler (amp 1), Plummer (amp 1, b 0.1), Milky Way},
+  method uniform over {DOPR54, DOP853}, span ~ U[5, 200], nt ~ U{11..501},
+  rtol = atol log-uniform on [1e-12, 1e-8]; near-circular disk orbits
+  (R ~ U[0.3, 2], z ~ N(0, 0.03), v_phi = vc(R)(1 + N(0, 0.05)),
+  v_R, v_z ~ N(0, 0.05)).  4,096 jobs (~32M orbits, ~400 GB of output) is the
+  full mix; the benchmark runs a prefix that fits one GPU.
 * ``mw_disk``: 1,048,576 orbits in the Milky-Way model (bulge alpha = 1.8,
   rc = 0.2375, vc^2 fraction 0.05; Miyamoto-Nagai a = 0.375, b = 0.035, 0.6;
   NFW a = 2.0, 0.35), exponential-disk initial conditions: R ~ R exp(-R/0.3125)
@@ -30,6 +37,7 @@ from ._native import PotentialSpec
 
 KEPLER_SEED = 0x1024
 MW_SEED = 0x100000
+RAGGED_SEED = 0x4096
 
 
 @dataclass
@@ -140,3 +148,56 @@ def tidal_stream(n: int, rng: np.random.Generator, span: float | None = None,
     t_end = span if span is not None else rng.uniform(10.0, 100.0)
     return Workload("tidal_stream", "DOP853", np.ascontiguousarray(state),
                     grid_times(0.0, t_end, steps), rtol, atol, spec)
+
+
+RAGGED_POTENTIALS = (PotentialSpec.kepler(1.0), PotentialSpec.plummer(1.0, 0.1),
+                     PotentialSpec.milky_way())
+
+
+def vcirc(spec: PotentialSpec, R: np.ndarray) -> np.ndarray:
+    """Circular speed in the z = 0 plane of a Kepler, Plummer or Milky-Way spec."""
+    R = np.asarray(R, dtype=np.float64)
+    if spec.kind == 1:  # Kepler
+        return np.sqrt(spec.amp / R)
+    if spec.kind == 2:  # Plummer
+        return np.sqrt(spec.amp * R * R / (R * R + spec.b * spec.b) ** 1.5)
+    return mw_vcirc(spec, R)
+
+
+@dataclass
+class RaggedJob:
+    method: str          # "DOPR54" or "DOP853"
+    potential: int       # index into RAGGED_POTENTIALS
+    state0: np.ndarray   # (n, 6)
+    times: np.ndarray    # (nt,)
+    rtol: float
+
+    @property
+    def n(self) -> int:
+        return self.state0.shape[0]
+
+    @property
+    def nt(self) -> int:
+        return self.times.shape[0]
+
+
+def ragged_mix(njobs: int = 4096, seed: int = RAGGED_SEED) -> list[RaggedJob]:
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(njobs):
+        n = int(np.exp(rng.uniform(math.log(16), math.log(65536))))
+        pot = int(rng.integers(0, 3))
+        method = "DOPR54" if rng.integers(0, 2) == 0 else "DOP853"
+        span = rng.uniform(5.0, 200.0)
+        nt = int(rng.integers(11, 502))
+        tol = math.exp(rng.uniform(math.log(1e-12), math.log(1e-8)))
+        R = rng.uniform(0.3, 2.0, n)
+        phi = rng.uniform(0.0, 2.0 * math.pi, n)
+        z = rng.normal(0.0, 0.03, n)
+        vphi = vcirc(RAGGED_POTENTIALS[pot], R) * (1.0 + rng.normal(0.0, 0.05, n))
+        vR, vz = rng.normal(0.0, 0.05, n), rng.normal(0.0, 0.05, n)
+        c, s = np.cos(phi), np.sin(phi)
+        state = np.stack([R * c, R * s, z, vR * c - vphi * s, vR * s + vphi * c, vz], axis=1)
+        jobs.append(RaggedJob(method, pot, np.ascontiguousarray(state),
+                              grid_times(0.0, span, nt), tol))
+    return jobs

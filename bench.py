@@ -48,6 +48,8 @@ def parse_args():
     p.add_argument("--serving-seconds", type=float, default=8.0,
                    help="length of the Poisson serving run (0 = skip)")
     p.add_argument("--serving-rate", type=float, default=8.0, help="requests per second")
+    p.add_argument("--ragged-jobs", type=int, default=2048,
+                   help="jobs of the ragged mix run as one batch (0 = skip)")
     return p.parse_args()
 
 
@@ -64,6 +66,52 @@ def profiled_traffic(n: int, schedule: str):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":", 1)[1])
     return None
+
+
+def ragged_leg(njobs: int):
+    """One drift_ragged_device call over the first `njobs` jobs of the ragged
+    mix (one persistent launch per integration method), inputs and outputs
+    resident in HBM; accepted particle-steps/s over the call."""
+    import numpy as np
+    import torch
+
+    from astrodrift_amd import _native, workloads
+
+    jobs = workloads.ragged_mix(njobs)
+    dev = torch.device("cuda:0")
+    keep, descs = [], []
+    for j in jobs:
+        s0 = torch.from_numpy(j.state0).to(dev)
+        tt = torch.from_numpy(j.times).to(dev)
+        out = torch.empty((j.nt, j.n, 6), dtype=torch.float64, device=dev)
+        st = torch.zeros((j.n, 4), dtype=torch.int32, device=dev)
+        keep.append((s0, tt, out, st))
+        lt = math.log(j.rtol)
+        method = _native.METHOD_DOP853 if j.method == "DOP853" else _native.METHOD_DOPR54
+        descs.append(_native.DeviceJob(method, j.potential, j.n, j.nt, lt, lt, s0.data_ptr(),
+                                       tt.data_ptr(), out.data_ptr(), st.data_ptr(),
+                                       span_hint=float(j.times[-1] - j.times[0])))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _native.ragged_device(descs, list(workloads.RAGGED_POTENTIALS))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    acc = rej = bad = 0
+    for (_, _, _, st) in keep:
+        s = st.cpu().numpy()
+        acc += int(s[:, 0].astype(np.int64).sum())
+        rej += int(s[:, 1].astype(np.int64).sum())
+        bad += int(np.count_nonzero(s[:, 3] & ~0xFF))
+    n = sum(j.n for j in jobs)
+    out_bytes = sum(48 * j.n * j.nt for j in jobs)
+    del keep
+    torch.cuda.empty_cache()
+    return {
+        "jobs": njobs, "orbits": n, "output_gb": round(out_bytes / 1e9, 2),
+        "dop853_jobs": sum(j.method == "DOP853" for j in jobs),
+        "accepted": acc, "rejected": rej, "flagged_orbits": bad,
+        "seconds": round(wall, 3), "value": acc / wall, "unit": UNIT,
+    }
 
 
 def cpu_baseline(w, stride: int):
@@ -213,6 +261,12 @@ def main() -> int:
         srv = loadgen.run_poisson(rate_hz=args.serving_rate, duration_s=args.serving_seconds)
         line["serving"] = srv
         print(f"[bench] serving: {srv}", file=sys.stderr, flush=True)
+    if args.ragged_jobs > 0:
+        if args.serving_seconds <= 0:
+            del out
+            torch.cuda.empty_cache()
+        line["ragged"] = ragged_leg(args.ragged_jobs)
+        print(f"[bench] ragged: {line['ragged']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, args.cpu_stride)
     print(json.dumps(line), flush=True)
