@@ -1,13 +1,16 @@
 // serving.cpp -- continuous-batching request loop over the ragged kernels.
 //
-// A server owns a session (device, stream, MW table cache) and one worker
-// thread.  Submitted requests queue first-come first-served; the worker admits
-// them while their device footprint fits the HBM budget, then runs chunk
+// A server owns a session (device, stream, MW table cache), a worker thread
+// that launches the chunks and a finisher thread that copies finished
+// requests out on a second stream.  Submitted requests queue first-come
+// first-served; the worker admits them while their device footprint fits the
+// HBM budget (beside what the finisher still holds), then runs chunk
 // launches: each advances every admitted request by up to chunk_samples
 // output samples (one persistent ragged launch per integration method), the
 // orbits' integrator state resident in HBM between chunks.  A request whose
-// orbits have all finished is copied to its host buffer and its device memory
-// released, which admits the next requests at the following chunk boundary.
+// orbits have all finished is handed to the finisher, which copies it to its
+// host buffer and releases its device memory while the next chunks run; the
+// memory released admits the next requests at the following chunk boundary.
 // So requests that arrive while others run join the GPU work within one
 // chunk, and every result is independent of what it shared the GPU with.
 #include <hip/hip_runtime.h>
@@ -70,15 +73,22 @@ struct drift_server {
     drift_session* session = nullptr;
     drift_server_options opt{};
     std::thread worker;
+    // copies finished requests to their host buffers and releases their
+    // device memory on its own stream, so chunk launches never wait for it
+    std::thread finisher;
+    hipStream_t copy_stream = nullptr;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_finish;
     std::deque<std::shared_ptr<drift::Request>> pending;
+    std::deque<std::shared_ptr<drift::Request>> finishing;
+    bool worker_done = false;
     std::map<uint64_t, std::shared_ptr<drift::Request>> requests;
     uint64_t next_ticket = 1;
     uint64_t used_bytes = 0;
     bool stopping = false;
     drift::Clock::time_point t0;
     drift::RaggedScratch scratch[2];
+    int32_t chunk_now = 8;  // samples per chunk (adaptive when opt.chunk_samples == 0)
 };
 
 namespace drift {
@@ -88,16 +98,24 @@ double now_s(const drift_server* S) {
     return std::chrono::duration<double>(Clock::now() - S->t0).count();
 }
 
-void complete(drift_server* S, Request& r, int state, int status) {
+void complete(drift_server* S, Request& r, int state, int status, uint64_t accepted = 0) {
     free_device(r);
     std::vector<double>().swap(r.state0);  // the server's input copies
     std::vector<double>().swap(r.times);
     std::lock_guard<std::mutex> lk(S->mu);
     S->used_bytes -= r.bytes;
+    r.info.accepted_steps = accepted;
     r.info.state = state;
     r.info.status = status;
     r.info.complete_s = now_s(S);
     S->cv_done.notify_all();
+    S->cv_work.notify_all();  // released memory may admit a queued request
+}
+
+// A queued request may start when it fits the budget beside what is held
+// (admitted or still being copied out), or when nothing is held at all.
+bool admissible(const drift_server* S, const Request& r) {
+    return S->used_bytes == 0 || S->used_bytes + r.bytes <= S->opt.hbm_budget_bytes;
 }
 
 // Allocates the request's device buffers and uploads its inputs.
@@ -124,9 +142,10 @@ int admit(drift_server* S, Request& r, hipStream_t st) {
 }
 
 // Copies a finished request to its host buffers.
-int finish(drift_server* S, Request& r, hipStream_t st, int& status) {
+int finish(Request& r, hipStream_t st, int& status, uint64_t& accepted) {
     const size_t n = r.req.n, nt = r.req.nt;
     status = DRIFT_OK;
+    accepted = 0;
     if (n == 0) return DRIFT_OK;
     std::vector<drift_orbit_stats> own;
     drift_orbit_stats* stats = r.req.stats;
@@ -145,7 +164,7 @@ int finish(drift_server* S, Request& r, hipStream_t st, int& status) {
         if (r.req.method == DRIFT_METHOD_DOPR54 && (stats[i].status & DRIFT_STATUS_CODE_MASK))
             status = DRIFT_ERR_INTEGRATION;
     }
-    r.info.accepted_steps = acc;
+    accepted = acc;
     return DRIFT_OK;
 }
 
@@ -170,7 +189,7 @@ int run_chunk(drift_server* S, std::vector<std::shared_ptr<Request>>& active, hi
     lo.precision = S->opt.precision;
     lo.block_size = S->opt.block_size;
     lo.max_attempts = S->opt.max_attempts;
-    const int32_t chunk = S->opt.chunk_samples > 0 ? S->opt.chunk_samples : 8;
+    const int32_t chunk = S->opt.chunk_samples > 0 ? S->opt.chunk_samples : S->chunk_now;
     for (int method : {DRIFT_METHOD_DOP853, DRIFT_METHOD_DOPR54}) {
         std::vector<size_t> idx;
         for (size_t k = 0; k < active.size(); ++k)
@@ -210,6 +229,7 @@ int run_chunk(drift_server* S, std::vector<std::shared_ptr<Request>>& active, hi
         DRIFT_HIP_TRY(hipMemcpyAsync(&r->host_done, r->d_done, sizeof(unsigned int),
                                      hipMemcpyDeviceToHost, st));
     DRIFT_HIP_TRY(hipStreamSynchronize(st));
+    std::lock_guard<std::mutex> lk(S->mu);
     for (auto& r : active) r->info.chunks++;
     return DRIFT_OK;
 }
@@ -223,12 +243,13 @@ void worker_main(drift_server* S) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
             if (active.empty())
-                S->cv_work.wait(lk, [&] { return S->stopping || !S->pending.empty(); });
+                S->cv_work.wait(lk, [&] {
+                    return (S->stopping && S->pending.empty()) ||
+                           (!S->pending.empty() && admissible(S, *S->pending.front()));
+                });
             if (S->stopping && S->pending.empty() && active.empty()) break;
-            while (!S->pending.empty()) {
+            while (!S->pending.empty() && admissible(S, *S->pending.front())) {
                 auto& r = S->pending.front();
-                const bool alone = active.empty() && admitted.empty();
-                if (!alone && S->used_bytes + r->bytes > S->opt.hbm_budget_bytes) break;
                 S->used_bytes += r->bytes;
                 admitted.push_back(r);
                 S->pending.pop_front();
@@ -245,21 +266,49 @@ void worker_main(drift_server* S) {
             }
         }
         if (active.empty()) continue;
+        const Clock::time_point c0 = Clock::now();
         int rc = run_chunk(S, active, st);
+        if (S->opt.chunk_samples == 0 && rc == DRIFT_OK) {
+            // adaptive chunk length: keep a chunk near the target duration so
+            // queued requests join within about one chunk, and chunks stay long
+            // enough that the pause/resume at their boundaries is cheap
+            const double ms = std::chrono::duration<double, std::milli>(Clock::now() - c0).count();
+            if (ms < 0.5 * S->opt.chunk_target_ms) S->chunk_now = std::min(S->chunk_now * 2, 512);
+            else if (ms > 2.0 * S->opt.chunk_target_ms) S->chunk_now = std::max(S->chunk_now / 2, 1);
+        }
         std::vector<std::shared_ptr<Request>> still;
         for (auto& r : active) {
             if (rc != DRIFT_OK) {
                 complete(S, *r, DRIFT_REQUEST_FAILED, rc);
             } else if (r->host_done >= r->req.n) {
-                int status = DRIFT_OK;
-                int frc = finish(S, *r, st, status);
-                complete(S, *r, frc == DRIFT_OK ? DRIFT_REQUEST_DONE : DRIFT_REQUEST_FAILED,
-                         frc == DRIFT_OK ? status : frc);
+                // the chunk's stream is synchronised: the outputs are final
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->finishing.push_back(r);
+                S->cv_finish.notify_one();
             } else {
                 still.push_back(r);
             }
         }
         active.swap(still);
+    }
+}
+
+void finisher_main(drift_server* S) {
+    (void)hipSetDevice(S->session->device);
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_finish.wait(lk, [&] { return S->worker_done || !S->finishing.empty(); });
+            if (S->finishing.empty()) break;  // worker_done and nothing left
+            r = S->finishing.front();
+            S->finishing.pop_front();
+        }
+        int status = DRIFT_OK;
+        uint64_t accepted = 0;
+        const int frc = finish(*r, S->copy_stream, status, accepted);
+        complete(S, *r, frc == DRIFT_OK ? DRIFT_REQUEST_DONE : DRIFT_REQUEST_FAILED,
+                 frc == DRIFT_OK ? status : frc, accepted);
     }
 }
 
@@ -278,6 +327,9 @@ int drift_server_create(int device, const drift_server_options* options, drift_s
     if (opt.precision != DRIFT_PRECISION_FP64 && opt.precision != DRIFT_PRECISION_MIXED)
         return fail(DRIFT_ERR_UNSUPPORTED, "unknown precision mode");
     if (opt.chunk_samples < 0) return fail(DRIFT_ERR_INVALID_ARGUMENT, "chunk_samples < 0");
+    if (!(opt.chunk_target_ms >= 0.0) || !std::isfinite(opt.chunk_target_ms))
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "chunk_target_ms must be finite and >= 0");
+    if (opt.chunk_target_ms == 0.0) opt.chunk_target_ms = 10.0;
     if (opt.block_size && (opt.block_size < 64 || opt.block_size > 256 || opt.block_size % 64))
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
     drift_session* sess = nullptr;
@@ -291,11 +343,18 @@ int drift_server_create(int device, const drift_server_options* options, drift_s
         }
         opt.hbm_budget_bytes = free_b / 2;
     }
+    hipStream_t copy_stream = nullptr;
+    if (hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking) != hipSuccess) {
+        drift_session_destroy(sess);
+        return fail(DRIFT_ERR_DEVICE, "hipStreamCreateWithFlags failed");
+    }
     drift_server* S = new drift_server();
     S->session = sess;
     S->opt = opt;
+    S->copy_stream = copy_stream;
     S->t0 = Clock::now();
     S->worker = std::thread(worker_main, S);
+    S->finisher = std::thread(finisher_main, S);
     *out = S;
     return DRIFT_OK;
 }
@@ -374,6 +433,13 @@ int drift_server_destroy(drift_server* S) {
     }
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->worker_done = true;
+    }
+    S->cv_finish.notify_all();
+    if (S->finisher.joinable()) S->finisher.join();
+    if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
     for (auto& sc : S->scratch) {
         if (sc.desc.ptr) (void)hipFree(sc.desc.ptr);
         if (sc.queue) (void)hipFree(sc.queue);

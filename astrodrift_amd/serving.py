@@ -42,6 +42,7 @@ class CServerOptions(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("block_size", ctypes.c_int32),
         ("max_attempts", ctypes.c_uint32),
+        ("chunk_target_ms", ctypes.c_double),
     ]
 
 
@@ -90,12 +91,14 @@ class Request:
 
 
 class Server:
-    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, chunk_samples: int = 8,
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, chunk_samples: int = 0,
                  precision: int = _native.PRECISION_FP64, block_size: int = 0,
-                 max_attempts: int = 0):
+                 max_attempts: int = 0, chunk_target_ms: float = 0.0):
+        """chunk_samples = 0 adapts the chunk length to chunk_target_ms (0 = 10 ms)."""
         self.lib = _native.load_library()
         _declare(self.lib)
-        opts = CServerOptions(hbm_budget_bytes, chunk_samples, precision, block_size, max_attempts)
+        opts = CServerOptions(hbm_budget_bytes, chunk_samples, precision, block_size, max_attempts,
+                              chunk_target_ms)
         handle = ctypes.c_void_p()
         _native.check(self.lib.drift_server_create(device, ctypes.byref(opts),
                                                    ctypes.byref(handle)))

@@ -219,10 +219,14 @@ typedef struct drift_request {
 
 typedef struct drift_server_options {
     uint64_t hbm_budget_bytes;    /* device bytes for admitted requests; 0 = half the free HBM */
-    int32_t chunk_samples;        /* output samples per orbit per chunk launch; 0 = 8 */
+    int32_t chunk_samples;        /* output samples per orbit per chunk launch; 0 = adaptive:
+                                     starts at 8 and is doubled or halved after each chunk to
+                                     keep a chunk's duration near chunk_target_ms, in [1, 512] */
     int32_t precision;            /* drift_precision for every request */
     int32_t block_size;           /* 0 = 256 */
     uint32_t max_attempts;        /* per-orbit attempt limit; 0 = default */
+    double chunk_target_ms;       /* adaptive chunks only; 0 = 10 ms.  A queued request waits
+                                     at most about one chunk before it joins the GPU work */
 } drift_server_options;
 
 typedef enum drift_request_state {
