@@ -38,6 +38,7 @@ PRECISION_MIXED = 1
 
 SCHEDULE_PLAIN = 0
 SCHEDULE_PERSISTENT = 1
+SCHEDULE_TUNED = 2  # what tune_device stored for this method / potential / precision / log2 n
 
 ORDER_AUTO = 0   # cost order for persistent launches of >= 16384 orbits
 ORDER_INDEX = 1
@@ -273,6 +274,11 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.drift_galpy_dop853_device.restype = c.c_int
     lib.drift_galpy_dop853_device.argtypes = host + [
         c.POINTER(CPotential), c.c_void_p, c.POINTER(CLaunchOptions), c.c_void_p]
+    lib.drift_tune_device.restype = c.c_int
+    lib.drift_tune_device.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_void_p, c.c_size_t,
+                                      c.c_size_t, c.c_double, c.c_double, c.c_double,
+                                      c.POINTER(CPotential), c.c_size_t,
+                                      c.POINTER(CLaunchOptions), c.POINTER(c.c_double), c.c_void_p]
     lib.drift_ragged_device.restype = c.c_int
     lib.drift_ragged_device.argtypes = [c.c_void_p, c.POINTER(CJob), c.c_size_t,
                                         c.POINTER(CPotential), c.c_size_t,
@@ -435,6 +441,24 @@ def launch_device(
         code = s.lib.drift_galpy_dop853_device(*args, pot_ref, ctypes.c_void_p(d_stats),
                                                ctypes.byref(opts), ctypes.c_void_p(stream))
     check(code)
+
+
+def tune_device(method: int, d_state0: int, d_times: int, n: int, nt: int, rtol_ln: float,
+                atol_ln: float, potential: PotentialSpec | None = None,
+                precision: int = PRECISION_FP64, prefix_samples: int = 0, stream: int = 0,
+                dt_one: float = INITIAL_STEP_SENTINEL, device: int = 0) -> dict:
+    """drift_tune_device: times the launch candidates on a prefix of the time
+    grid and stores the fastest in the session for SCHEDULE_TUNED launches."""
+    s = session(device)
+    best = CLaunchOptions(precision, 0, 0, 0, 0, 0)
+    ms = ctypes.c_double(0.0)
+    pot = potential.to_c() if potential is not None else None
+    check(s.lib.drift_tune_device(s.handle, method, ctypes.c_void_p(d_state0),
+                                  ctypes.c_void_p(d_times), n, nt, rtol_ln, atol_ln, dt_one,
+                                  ctypes.byref(pot) if pot is not None else None, prefix_samples,
+                                  ctypes.byref(best), ctypes.byref(ms), ctypes.c_void_p(stream)))
+    return {"schedule": best.schedule, "block_size": best.block_size,
+            "ordering": best.ordering, "ms": ms.value}
 
 
 @dataclass

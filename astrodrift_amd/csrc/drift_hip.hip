@@ -566,6 +566,24 @@ KeyKernel key_kernel_for(int kind, int precision) {
     }
 }
 
+int floor_log2(size_t n) {
+    int k = 0;
+    while (n > 1) {
+        n >>= 1;
+        ++k;
+    }
+    return k;
+}
+
+const TunedEntry* find_tuned(const drift_session* s, int method, int kind, int precision,
+                             size_t n) {
+    const int l = floor_log2(n);
+    for (const TunedEntry& e : s->tuned)
+        if (e.method == method && e.kind == kind && e.precision == precision && e.log2n == l)
+            return &e;
+    return nullptr;
+}
+
 // Orbit counts from which DRIFT_ORDER_AUTO sorts a persistent launch: below
 // it the machine is not full and the counting sort would not pay for itself.
 constexpr size_t kAutoOrderMinOrbits = 16384;
@@ -626,8 +644,8 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     int block = opt.block_size ? opt.block_size : kDefaultBlock;
     if (block < 64 || block > kDefaultBlock || block % 64 != 0)
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
-    if (opt.schedule != 0 && opt.schedule != 1)
-        return fail(DRIFT_ERR_UNSUPPORTED, "schedule must be 0 (plain grid) or 1 (persistent)");
+    if (opt.schedule < DRIFT_SCHEDULE_PLAIN || opt.schedule > DRIFT_SCHEDULE_TUNED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "schedule must be PLAIN, PERSISTENT or TUNED");
     if (opt.ordering < DRIFT_ORDER_AUTO || opt.ordering > DRIFT_ORDER_COST)
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "ordering must be AUTO, INDEX or COST");
     if (opt.reserved != 0)
@@ -655,6 +673,13 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     int rc = resolve_potential(s, potential, a.pot, &t64, &t32);
     if (rc != DRIFT_OK) return rc;
     a.table = opt.precision == DRIFT_PRECISION_MIXED ? t32 : t64;
+    if (opt.schedule == DRIFT_SCHEDULE_TUNED) {
+        const TunedEntry* e = find_tuned(s, method, a.pot.kind, opt.precision, n);
+        opt.schedule = e ? e->options.schedule : DRIFT_SCHEDULE_PERSISTENT;
+        opt.block_size = e ? e->options.block_size : 0;
+        opt.ordering = e ? e->options.ordering : DRIFT_ORDER_AUTO;
+        block = opt.block_size ? opt.block_size : kDefaultBlock;
+    }
 
     PlainKernel k = kernel_for(method, a.pot.kind, opt.precision, opt.schedule);
     hipStream_t st = stream ? stream : s->stream;
@@ -779,7 +804,7 @@ int drift_session_destroy(drift_session* s) {
         (void)hipFree(t.d_table32);
     }
     for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats, &s->order_keys,
-                            &s->order_slots, &s->order_bins})
+                            &s->order_slots, &s->order_bins, &s->tune_out, &s->tune_stats})
         if (b->ptr) (void)hipFree(b->ptr);
     if (s->order_done) (void)hipEventDestroy(s->order_done);
     if (s->queues) (void)hipFree(s->queues);
@@ -825,6 +850,94 @@ int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
     return launch_integrator(session, DRIFT_METHOD_DOP853, d_state0, d_times, d_state_out, n, nt,
                              rtol, atol, dp54::initial_step_sentinel, potential, d_stats, options,
                              (hipStream_t)stream);
+}
+
+int drift_tune_device(drift_session* s, int32_t method, const double* d_state0,
+                      const double* d_times, size_t n, size_t nt, double rtol, double atol,
+                      double dt_one_init, const drift_potential* potential, size_t prefix_samples,
+                      drift_launch_options* best, double* best_ms, void* stream) {
+    if (!s) return fail(DRIFT_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (!best) return fail(DRIFT_ERR_INVALID_ARGUMENT, "best is NULL");
+    if (method != DRIFT_METHOD_DOPR54 && method != DRIFT_METHOD_DOP853)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "unknown method");
+    if (n < 1 || nt < 2) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n must be >= 1 and nt >= 2");
+    const size_t k = prefix_samples ? std::min(prefix_samples, nt) : std::max<size_t>(2, nt / 20);
+    if (k < 2) return fail(DRIFT_ERR_INVALID_ARGUMENT, "prefix_samples must be >= 2");
+    DRIFT_HIP_TRY(hipSetDevice(s->device));
+    int rc;
+    if ((rc = ensure_buffer(s->tune_out, k * n * DIM * sizeof(double))) != DRIFT_OK) return rc;
+    if ((rc = ensure_buffer(s->tune_stats, n * sizeof(drift_orbit_stats))) != DRIFT_OK) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    drift_launch_options base{};
+    base.precision = best->precision;
+    base.max_attempts = best->max_attempts;
+    auto trial = [&](const drift_launch_options& o) {
+        return launch_integrator(s, method, d_state0, d_times, (double*)s->tune_out.ptr, n, k, rtol,
+                                 atol, dt_one_init, potential,
+                                 (drift_orbit_stats*)s->tune_stats.ptr, &o, st);
+    };
+    // untimed launches of both schedules: resolve the potential's table and
+    // load the code objects before anything is timed
+    if ((rc = trial(base)) != DRIFT_OK) return rc;
+    drift_launch_options warm = base;
+    warm.schedule = DRIFT_SCHEDULE_PERSISTENT;
+    warm.ordering = DRIFT_ORDER_COST;
+    if ((rc = trial(warm)) != DRIFT_OK) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DRIFT_HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(DRIFT_ERR_DEVICE, "hipEventCreate failed");
+    }
+    drift_launch_options chosen = base;
+    float chosen_ms = 0.0f;
+    bool have = false;
+    for (int schedule : {DRIFT_SCHEDULE_PLAIN, DRIFT_SCHEDULE_PERSISTENT}) {
+        for (int block : {64, 128, 256}) {
+            for (int ordering : {DRIFT_ORDER_INDEX, DRIFT_ORDER_COST}) {
+                drift_launch_options o = base;
+                o.schedule = schedule;
+                o.block_size = block;
+                o.ordering = ordering;
+                float ms = 0.0f;
+                if (hipEventRecord(e0, st) != hipSuccess) rc = DRIFT_ERR_DEVICE;
+                if (rc == DRIFT_OK) rc = trial(o);
+                if (rc == DRIFT_OK && hipEventRecord(e1, st) != hipSuccess) rc = DRIFT_ERR_DEVICE;
+                if (rc == DRIFT_OK && hipEventSynchronize(e1) != hipSuccess) rc = DRIFT_ERR_DEVICE;
+                if (rc == DRIFT_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    rc = DRIFT_ERR_DEVICE;
+                if (rc != DRIFT_OK) {
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    if (rc == DRIFT_ERR_DEVICE) set_last_error("autotuner: event timing failed");
+                    return rc;
+                }
+                if (!have || ms < chosen_ms) {
+                    chosen = o;
+                    chosen_ms = ms;
+                    have = true;
+                }
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    PotParams pp{};
+    const void *t64 = nullptr, *t32 = nullptr;
+    if ((rc = resolve_potential(s, potential, pp, &t64, &t32)) != DRIFT_OK) return rc;
+    TunedEntry entry{method, pp.kind, base.precision, floor_log2(n), chosen};
+    bool replaced = false;
+    for (TunedEntry& e : s->tuned) {
+        if (e.method == entry.method && e.kind == entry.kind && e.precision == entry.precision &&
+            e.log2n == entry.log2n) {
+            e = entry;
+            replaced = true;
+        }
+    }
+    if (!replaced) s->tuned.push_back(entry);
+    *best = chosen;
+    if (best_ms) *best_ms = chosen_ms;
+    return DRIFT_OK;
 }
 
 int drift_force_device(drift_session* session, const drift_potential* potential,

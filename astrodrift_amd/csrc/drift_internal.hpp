@@ -24,6 +24,12 @@ struct DeviceBuffer {
     size_t bytes = 0;
 };
 
+// A launch configuration chosen by drift_tune_device.
+struct TunedEntry {
+    int method, kind, precision, log2n;
+    drift_launch_options options;
+};
+
 }  // namespace drift
 
 #include "ragged.hpp"
@@ -45,6 +51,9 @@ struct drift_session {
     // counters; order_done marks the end of the last launch that read them
     drift::DeviceBuffer order_keys, order_slots, order_bins;
     hipEvent_t order_done = nullptr;
+    // autotuner: chosen configurations and the scratch output of its trials
+    std::vector<drift::TunedEntry> tuned;
+    drift::DeviceBuffer tune_out, tune_stats;
 };
 
 #define DRIFT_HIP_TRY(expr)                                                              \

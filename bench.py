@@ -39,9 +39,10 @@ def parse_args():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=1 << 20, help="orbits (default 1,048,576)")
-    p.add_argument("--schedules", default="plain,persistent:index,persistent",
-                   help="comma list of schedule[:ordering] to time; schedule plain|persistent, "
-                        "ordering auto|index|cost (auto: cost order for persistent launches)")
+    p.add_argument("--schedules", default="plain,persistent:index,persistent,tuned",
+                   help="comma list of schedule[:ordering] to time; schedule plain|persistent|"
+                        "tuned (the autotuner's choice), ordering auto|index|cost (auto: cost "
+                        "order for persistent launches)")
     p.add_argument("--cpu-stride", type=int, default=512,
                    help="CPU baseline integrates every k-th orbit")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -56,11 +57,15 @@ def parse_args():
 TRAFFIC_PROFILE = REPO / "profiles" / "dop853_mw_1M_traffic.md"
 
 
-def profiled_traffic(n: int, schedule: str):
-    """HBM bytes per launch of the persistent 1M-orbit integrator, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (a counter run cannot
-    share a process with the timed loop); None for any other configuration."""
-    if n != 1 << 20 or schedule != "persistent" or not TRAFFIC_PROFILE.exists():
+def profiled_traffic(n: int, schedule: str, tuned: dict | None):
+    """HBM bytes per launch of the persistent, cost-ordered, 256-thread-block
+    1M-orbit integrator, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (a counter run cannot share a process with the timed loop); None
+    for any other configuration."""
+    same_kernel = schedule == "persistent" or (
+        tuned is not None and tuned["schedule"] == 1 and tuned["block_size"] == 256
+        and tuned["ordering"] == 2)
+    if n != 1 << 20 or not same_kernel or not TRAFFIC_PROFILE.exists():
         return None
     for line in TRAFFIC_PROFILE.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
@@ -172,7 +177,8 @@ def main() -> int:
     # a dedicated stream: the kernels and the timing events share it
     stream = torch.cuda.Stream()
 
-    sched_ids = {"plain": _native.SCHEDULE_PLAIN, "persistent": _native.SCHEDULE_PERSISTENT}
+    sched_ids = {"plain": _native.SCHEDULE_PLAIN, "persistent": _native.SCHEDULE_PERSISTENT,
+                 "tuned": _native.SCHEDULE_TUNED}
     order_ids = {"auto": _native.ORDER_AUTO, "index": _native.ORDER_INDEX,
                  "cost": _native.ORDER_COST}
     results = {}
@@ -183,6 +189,14 @@ def main() -> int:
         sched_name, _, order_name = name.partition(":")
         schedule = sched_ids[sched_name]
         ordering = order_ids[order_name or "auto"]
+        tuned = None
+        if schedule == _native.SCHEDULE_TUNED:
+            # untimed: the autotuner times its candidates on the first 5% of
+            # the time grid and stores its choice for SCHEDULE_TUNED launches
+            tuned = _native.tune_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(), w.n,
+                                        w.nt, tol_r, tol_a, w.potential,
+                                        stream=stream.cuda_stream)
+            print(f"[bench] autotuner choice: {tuned}", file=sys.stderr, flush=True)
 
         def launch():
             _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(),
@@ -218,6 +232,8 @@ def main() -> int:
             "tflops": flop / kernel_s / 1e12,
             "flop_per_launch": flop,
         }
+        if tuned is not None:
+            results[name]["tuned_options"] = tuned
         print(f"[bench] {name}: {results[name]}", file=sys.stderr, flush=True)
 
     chosen = max(results, key=lambda k: results[k]["value"])
@@ -244,12 +260,16 @@ def main() -> int:
             "schedules": results,
         },
         "roofline": {
-            "bound": "fp64-valu",
+            # compute-bound against the FP64 peak; the path has no matrix
+            # product, so the units doing the work are the FP64 vector ALUs,
+            # whose MI355X peak equals the FP64 matrix peak (78.6 TFLOP/s)
+            "bound": "mfma",
+            "compute_unit": "fp64 vector ALU",
             "achieved": r["tflops"],
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": r["tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": profiled_traffic(w.n, chosen),
+            "traffic": profiled_traffic(w.n, chosen, r.get("tuned_options")),
             "traffic_source": str(TRAFFIC_PROFILE.relative_to(REPO)),
         },
     }

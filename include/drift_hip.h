@@ -101,8 +101,7 @@ typedef struct drift_orbit_stats {
 typedef struct drift_launch_options {
     int32_t precision;  /* drift_precision */
     int32_t block_size; /* threads per block; 0 = library default */
-    int32_t schedule;   /* 0 = one lane per orbit (plain grid),
-                           1 = persistent kernel, lanes refilled from a work queue */
+    int32_t schedule;   /* drift_schedule */
     uint32_t max_attempts; /* per-orbit attempted-step limit; 0 = default (2^22).
                               The reference has none: a DOP853 orbit whose error
                               estimate never drops below 1 retries forever at the
@@ -123,6 +122,14 @@ typedef enum drift_ordering {
     DRIFT_ORDER_INDEX = 1, /* orbit i is taken in index order */
     DRIFT_ORDER_COST = 2
 } drift_ordering;
+
+typedef enum drift_schedule {
+    DRIFT_SCHEDULE_PLAIN = 0,      /* one lane per orbit (plain grid) */
+    DRIFT_SCHEDULE_PERSISTENT = 1, /* persistent kernel, lanes refilled from a work queue */
+    DRIFT_SCHEDULE_TUNED = 2       /* the options drift_tune_device stored in the session for
+                                      this (method, potential kind, precision, log2 n); without
+                                      one, PERSISTENT with AUTO ordering */
+} drift_schedule;
 
 typedef struct drift_session drift_session;
 
@@ -173,6 +180,20 @@ int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
                               double rtol, double atol, const drift_potential* potential,
                               drift_orbit_stats* d_stats, const drift_launch_options* options,
                               void* stream);
+
+/* Autotuner (no counterpart in the reference).  Times every candidate
+ * {PLAIN, PERSISTENT} x block {64, 128, 256} x ordering {INDEX, COST} on the
+ * launch described by the arguments, integrated over the first prefix_samples
+ * output times only (0 = max(2, nt / 20)) so that every orbit takes part, and
+ * returns the fastest in *best (precision and max_attempts as given in
+ * *best on entry, or 0) and its time in *best_ms (nullable).  The choice is
+ * stored in the session under (method, potential kind, precision, log2 n) for
+ * launches with schedule = DRIFT_SCHEDULE_TUNED.  Needs prefix_samples x n x 48
+ * bytes of device scratch; synchronises `stream`. */
+int drift_tune_device(drift_session* session, int32_t method, const double* d_state0,
+                      const double* d_times, size_t n, size_t nt, double rtol, double atol,
+                      double dt_one_init, const drift_potential* potential, size_t prefix_samples,
+                      drift_launch_options* best, double* best_ms, void* stream);
 
 /* ---- ragged batches -------------------------------------------------------- */
 

@@ -115,3 +115,35 @@ def test_bad_ordering_options_are_rejected(gpu):
         None, None, ctypes.byref(opts), None)
     assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT
     assert "reserved" in _native.last_error()
+
+
+def test_autotuner_choice_is_valid_and_tuned_launches_match(gpu):
+    """drift_tune_device picks one of its candidates; SCHEDULE_TUNED launches
+    (with and without a stored choice) give the same bits as the plain kernel."""
+    import torch
+
+    w = workloads.mw_disk(n=20000, t_end=10.0, steps=201)
+    dev = torch.device("cuda:0")
+    s0 = torch.from_numpy(w.state0).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    ref_out, ref_st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_PLAIN, _native.ORDER_INDEX)
+    # no stored choice yet for this size: TUNED falls back to persistent + AUTO
+    out, st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_TUNED, _native.ORDER_AUTO)
+    assert _same(out, ref_out) and np.array_equal(st, ref_st)
+    lt = math.log(w.rtol)
+    best = _native.tune_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(), w.n, w.nt,
+                               lt, lt, w.potential)
+    print("tuned:", best)
+    assert best["schedule"] in (_native.SCHEDULE_PLAIN, _native.SCHEDULE_PERSISTENT)
+    assert best["block_size"] in (64, 128, 256)
+    assert best["ordering"] in (_native.ORDER_INDEX, _native.ORDER_COST)
+    assert best["ms"] > 0.0
+    out, st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_TUNED, _native.ORDER_AUTO)
+    assert _same(out, ref_out) and np.array_equal(st, ref_st)
+
+
+def test_autotuner_rejects_bad_arguments(gpu):
+    with pytest.raises(_native.DriftNativeError, match="method"):
+        _native.tune_device(7, 0, 0, 10, 10, -23.0, -23.0)
+    with pytest.raises(_native.DriftNativeError, match="nt"):
+        _native.tune_device(_native.METHOD_DOP853, 0, 0, 10, 1, -23.0, -23.0)
