@@ -584,6 +584,29 @@ const TunedEntry* find_tuned(const drift_session* s, int method, int kind, int p
     return nullptr;
 }
 
+bool table_cached(const drift_session* s, const drift_potential& p) {
+    for (const auto& e : s->tables)
+        if (same_potential(e.key, p)) return true;
+    return false;
+}
+
+// Picks the next work-queue counter of the ring.  Eager launches rotate over
+// the counters no captured graph owns (stream order retires a counter's
+// previous user before the ring comes back to it); a captured launch takes
+// one for the life of the session, as its graph may be replayed at any time.
+// A few counters always stay free for eager launches.
+int take_queue_slot(drift_session* s, bool capturing) {
+    const unsigned owned = (unsigned)std::count(s->queue_owned.begin(), s->queue_owned.end(), true);
+    if (capturing && owned + 8 >= kQueueRing) return -1;
+    for (unsigned k = 0; k < kQueueRing; ++k) {
+        const unsigned slot = (s->queue_next++) % kQueueRing;
+        if (s->queue_owned[slot]) continue;
+        if (capturing) s->queue_owned[slot] = true;
+        return (int)slot;
+    }
+    return -1;
+}
+
 // Orbit counts from which DRIFT_ORDER_AUTO sorts a persistent launch: below
 // it the machine is not full and the counting sort would not pay for itself.
 constexpr size_t kAutoOrderMinOrbits = 16384;
@@ -617,7 +640,6 @@ int build_cost_order(drift_session* s, const PlainArgs& a, int precision, hipStr
     return DRIFT_OK;
 }
 
-constexpr unsigned kQueueRing = 64;
 
 constexpr uint32_t kDefaultMaxAttempts = 1u << 22;
 constexpr double kMixedMinRtol = 1e-7;
@@ -669,6 +691,17 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
                     "mixed precision (FP32 stage forces) requires rtol >= 1e-7");
 
     DRIFT_HIP_TRY(hipSetDevice(s->device));
+    hipStream_t st = stream ? stream : s->stream;
+    // Under stream capture (hipGraph) nothing may allocate or synchronise:
+    // the potential's table and the queue ring must exist from an earlier
+    // eager launch, and the launch owns its queue counter for good.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    DRIFT_HIP_TRY(hipStreamIsCapturing(st, &capture));
+    const bool capturing = capture != hipStreamCaptureStatusNone;
+    if (capturing && potential && potential->kind == POT_MW && !table_cached(s, *potential))
+        return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                    "stream capture: run one eager launch with this potential on this session "
+                    "first (it builds and uploads the force table)");
     const void *t64 = nullptr, *t32 = nullptr;
     int rc = resolve_potential(s, potential, a.pot, &t64, &t32);
     if (rc != DRIFT_OK) return rc;
@@ -681,12 +714,27 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         block = opt.block_size ? opt.block_size : kDefaultBlock;
     }
 
+    if (capturing && opt.ordering == DRIFT_ORDER_COST)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "cost ordering uses per-session scratch and cannot be captured into a graph; "
+                    "pass DRIFT_ORDER_INDEX (AUTO means INDEX under capture)");
     PlainKernel k = kernel_for(method, a.pot.kind, opt.precision, opt.schedule);
-    hipStream_t st = stream ? stream : s->stream;
     int64_t blocks = ((int64_t)n + block - 1) / block;
     if (opt.schedule == 1) {
-        if (!s->queues) DRIFT_HIP_TRY(hipMalloc(&s->queues, kQueueRing * sizeof(unsigned long long)));
-        a.queue = s->queues + (s->queue_next++ % kQueueRing);
+        if (!s->queues) {
+            if (capturing)
+                return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                            "stream capture: run one eager persistent launch on this session "
+                            "first (it allocates the work-queue counters)");
+            DRIFT_HIP_TRY(hipMalloc(&s->queues, kQueueRing * sizeof(unsigned long long)));
+            s->queue_owned.assign(kQueueRing, false);
+        }
+        const int slot = take_queue_slot(s, capturing);
+        if (slot < 0)
+            return fail(DRIFT_ERR_UNSUPPORTED,
+                        "too many captured persistent launches on this session (each graph keeps "
+                        "its own work-queue counter)");
+        a.queue = s->queues + slot;
         int per_cu = 0;
         DRIFT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0));
         const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(s->cu_count, 1);
@@ -696,8 +744,9 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     }
     if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");
     const bool cost_order =
-        opt.ordering == DRIFT_ORDER_COST ||
-        (opt.ordering == DRIFT_ORDER_AUTO && opt.schedule == 1 && n >= kAutoOrderMinOrbits);
+        !capturing &&
+        (opt.ordering == DRIFT_ORDER_COST ||
+         (opt.ordering == DRIFT_ORDER_AUTO && opt.schedule == 1 && n >= kAutoOrderMinOrbits));
     if (cost_order) {
         if (n > (size_t)UINT32_MAX)
             return fail(DRIFT_ERR_INVALID_ARGUMENT, "cost ordering supports n < 2^32");

@@ -45,6 +45,7 @@ struct drift_session {
     // stream order has retired its previous user)
     unsigned long long* queues = nullptr;
     unsigned queue_next = 0;
+    std::vector<bool> queue_owned;  // counters held by captured graphs
     // descriptor buffers of drift_ragged_device, one per method
     drift::RaggedScratch ragged[2];
     // cost ordering: per-orbit bin keys, the slot -> orbit map and the bin

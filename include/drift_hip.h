@@ -175,6 +175,14 @@ int drift_galpy_dopr54_device(drift_session* session, const double* d_state0,
                               const drift_potential* potential, drift_orbit_stats* d_stats,
                               const drift_launch_options* options, void* stream);
 
+/* Stream capture: the two _device entry points may be captured into a
+ * hipGraph (hipStreamBeginCapture on `stream`) after one eager launch of the
+ * same session with the same potential and schedule (which builds the force
+ * table and the work-queue counters; capture allocates nothing).  Under
+ * capture DRIFT_ORDER_AUTO means INDEX and DRIFT_ORDER_COST is refused; each
+ * captured persistent launch keeps a work-queue counter of the session for
+ * good (up to 56 per session).  A graph must not be replayed concurrently
+ * with itself. */
 int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
                               const double* d_times, double* d_state_out, size_t n, size_t nt,
                               double rtol, double atol, const drift_potential* potential,
