@@ -1,0 +1,105 @@
+"""The full 1,048,576-orbit Milky-Way DOP853 workload, checked through
+properties that do not depend on its size:
+
+* sample 0 of every orbit is its initial state, bit for bit;
+* every orbit ends with status 0 and finite samples;
+* energy and the z angular momentum (the model is axisymmetric and static)
+  are conserved to the tolerance's order over t = 0..50, with the energy
+  taken from an independent closed form of the potential (scipy's
+  incomplete gamma functions for the bulge), not from the oracle;
+* 64 orbits drawn at random match the oracle over every sample (1e-8).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from astrodrift_amd import _native, workloads
+
+import oracle_lib
+
+
+def mw_potential(spec, x, y, z):
+    """Phi of the Milky-Way spec (G = 1): power-law bulge with a Gaussian
+    cutoff, Miyamoto-Nagai disk, NFW halo."""
+    from scipy.special import gamma as gamma_fn
+    from scipy.special import gammainc, gammaincc
+
+    r2 = x * x + y * y + z * z
+    r = np.sqrt(r2)
+    a_lo = 1.5 - 0.5 * spec.bulge_alpha
+    a_hi = 1.0 - 0.5 * spec.bulge_alpha
+    xx = r2 / spec.bulge_rc ** 2
+    mass = (2.0 * math.pi * spec.bulge_amp * spec.bulge_rc ** (3.0 - spec.bulge_alpha)
+            * gammainc(a_lo, xx) * gamma_fn(a_lo))
+    outer = (2.0 * math.pi * spec.bulge_amp * spec.bulge_rc ** (2.0 - spec.bulge_alpha)
+             * gammaincc(a_hi, xx) * gamma_fn(a_hi))
+    phi_bulge = -mass / r - outer
+    zb = np.sqrt(z * z + spec.disk_b ** 2)
+    phi_disk = -spec.disk_amp / np.sqrt(x * x + y * y + (spec.disk_a + zb) ** 2)
+    phi_halo = -spec.halo_amp * np.log1p(r / spec.halo_a) / r
+    return phi_bulge + phi_disk + phi_halo
+
+
+def test_potential_closed_form_matches_the_oracle():
+    """The closed form used for the energies agrees with the oracle's potential."""
+    spec = _native.PotentialSpec.milky_way()
+    rng = np.random.default_rng(11)
+    pts = rng.normal(0.0, 1.0, (32, 3))
+    mine = mw_potential(spec, pts[:, 0], pts[:, 1], pts[:, 2])
+    ref = np.array([oracle_lib.potential_value(spec, *p) for p in pts])
+    np.testing.assert_allclose(mine, ref, rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_full_mw_workload_properties(gpu):
+    import torch
+
+    w = workloads.mw_disk()
+    dev = torch.device("cuda:0")
+    s0 = torch.from_numpy(w.state0).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    out = torch.empty((w.nt, w.n, 6), dtype=torch.float64, device=dev)
+    st = torch.zeros((w.n, 4), dtype=torch.int32, device=dev)
+    lt = math.log(w.rtol)
+    _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(), out.data_ptr(),
+                          w.n, w.nt, lt, lt, w.potential, st.data_ptr(),
+                          schedule=_native.SCHEDULE_PERSISTENT)
+    torch.cuda.synchronize()
+    first = out[0].cpu().numpy()
+    last = out[-1].cpu().numpy()
+    stats = st.cpu().numpy()
+    rng = np.random.default_rng(2024)
+    pick = np.sort(rng.choice(w.n, 64, replace=False))
+    traj = out[:, torch.from_numpy(pick).to(dev)].cpu().numpy()
+    finite = bool(torch.isfinite(out).all().item())
+    del out
+    torch.cuda.empty_cache()
+
+    assert np.array_equal(first.view(np.uint64), w.state0.view(np.uint64))
+    assert finite
+    assert not stats[:, 3].any()
+    spec = w.potential
+    e0 = 0.5 * np.sum(first[:, 3:] ** 2, 1) + mw_potential(spec, *first[:, :3].T)
+    e1 = 0.5 * np.sum(last[:, 3:] ** 2, 1) + mw_potential(spec, *last[:, :3].T)
+    de = np.abs((e1 - e0) / e0)
+    lz0 = first[:, 0] * first[:, 4] - first[:, 1] * first[:, 3]
+    lz1 = last[:, 0] * last[:, 4] - last[:, 1] * last[:, 3]
+    dl = np.abs(lz1 - lz0) / np.abs(lz0)
+    q = [0.5, 0.99, 0.9999, 1.0]
+    print("accepted", int(stats[:, 0].astype(np.int64).sum()),
+          "|dE/E| quantiles", np.quantile(de, q), "|dLz/Lz| quantiles", np.quantile(dl, q))
+    assert np.median(de) < 1e-11 and np.quantile(de, 0.9999) < 1e-9 and de.max() < 1e-9
+    assert np.median(dl) < 1e-11 and np.quantile(dl, 0.9999) < 1e-9 and dl.max() < 1e-8
+
+    cpu, cst = oracle_lib.batch(_native.METHOD_DOP853, np.ascontiguousarray(w.state0[pick]),
+                                w.times, lt, lt, spec, nthreads=8)
+    dp = np.linalg.norm(traj[..., :3] - cpu[..., :3], axis=-1) / np.linalg.norm(cpu[..., :3], axis=-1)
+    dv = np.linalg.norm(traj[..., 3:] - cpu[..., 3:], axis=-1) / np.linalg.norm(cpu[..., 3:], axis=-1)
+    print(f"64 random orbits: pos_rel={dp.max():.3e} vel_rel={dv.max():.3e}")
+    assert dp.max() < 1e-8 and dv.max() < 1e-8
+    acc_g = int(stats[pick, 0].astype(np.int64).sum())
+    assert abs(acc_g - int(cst["accepted"].sum())) <= 1e-3 * acc_g
