@@ -36,12 +36,20 @@ enum : int32_t {
 // index from the binary exponent of r2 and whether its mantissa is >= sqrt(2),
 // TABLE_DEG+1 monomial coefficients per segment in the local variable
 // s = (m - centre) * inv_half in [-1, 1], m = r2 / 2^e in [1, 2).
+// The LDS table covers [2^-64, 2^32) (r from 2.3e-10 to 65,536).  A second,
+// "far" table of the same layout covers every positive finite double r2,
+// [2^-1074, 2^1024), and stays in global memory: it is read only when an
+// orbit leaves the LDS range, with the same few registers as the LDS path.
 constexpr int TABLE_E_MIN = -64;
 constexpr int TABLE_E_MAX = 32;
 constexpr int TABLE_DEG = 15;
 constexpr int TABLE_NCOEF = TABLE_DEG + 1;
 constexpr int TABLE_SEGMENTS = 2 * (TABLE_E_MAX - TABLE_E_MIN);
 constexpr int TABLE_DOUBLES = TABLE_SEGMENTS * TABLE_NCOEF;
+constexpr int FAR_E_MIN = -1074;
+constexpr int FAR_E_MAX = 1024;
+constexpr int FAR_SEGMENTS = 2 * (FAR_E_MAX - FAR_E_MIN);
+constexpr int FAR_DOUBLES = FAR_SEGMENTS * TABLE_NCOEF;
 
 // Potential parameters in the form the kernels use (derived on the host
 // from drift_potential).
@@ -51,12 +59,7 @@ struct PotParams {
     double amp;       // Kepler / Plummer amplitude
     double b2;        // Plummer b^2
     double disk_amp, disk_a, disk_b2;
-    // spherical part outside the table range (slow path)
-    double bulge_mass_total;   // 2 pi amp rc^(3-alpha) Gamma(3/2 - alpha/2)
-    double bulge_small_c0;     // S_b(r2) ~ r2^(-alpha/2) * (c0 + c1 * r2) for small r2
-    double bulge_small_c1;
-    double bulge_exponent;     // -alpha/2
-    double halo_amp, halo_a;
+    const double* far_table;  // MW: FAR_DOUBLES coefficients in device memory
 };
 
 DRIFT_HD DRIFT_FI double rsqrt_d(double x) {
@@ -75,14 +78,15 @@ DRIFT_HD DRIFT_FI float rsqrt_f(float x) {
 #endif
 }
 
-// Segment index and local coordinate of r2 in the spherical table.  Returns
-// false when r2 is outside [2^E_MIN, 2^E_MAX) (or not finite).
-DRIFT_HD DRIFT_FI bool table_locate(double r2, int& seg, double& s) {
+// Segment index and local coordinate of r2 in a table of segments
+// [2^E0, 2^E1).  Returns false when r2 is outside it.
+template <int E0, int E1>
+DRIFT_HD DRIFT_FI bool table_locate_in(double r2, int& seg, double& s) {
     int e;
     double m = std::frexp(r2, &e);  // r2 = m 2^e, m in [0.5, 1)
     m *= 2.0;                        // m in [1, 2)
     e -= 1;
-    if (!(e >= TABLE_E_MIN && e < TABLE_E_MAX)) return false;
+    if (!(e >= E0 && e < E1)) return false;
     const double kSqrt2 = 1.4142135623730951;
     int half = m >= kSqrt2 ? 1 : 0;
     // half 0: [1, sqrt2), half 1: [sqrt2, 2)
@@ -91,8 +95,13 @@ DRIFT_HD DRIFT_FI bool table_locate(double r2, int& seg, double& s) {
     const double centre = 0.5 * (lo + hi);
     const double inv_half = 2.0 / (hi - lo);
     s = (m - centre) * inv_half;
-    seg = 2 * (e - TABLE_E_MIN) + half;
+    seg = 2 * (e - E0) + half;
     return true;
+}
+
+// The LDS table's range (false outside [2^E_MIN, 2^E_MAX) or not finite).
+DRIFT_HD DRIFT_FI bool table_locate(double r2, int& seg, double& s) {
+    return table_locate_in<TABLE_E_MIN, TABLE_E_MAX>(r2, seg, s);
 }
 
 template <class T>
@@ -103,62 +112,15 @@ DRIFT_HD DRIFT_FI T horner_table(const T* c, T s) {
     return acc;
 }
 
-// ln x for finite x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
-// ln m = 2 atanh(z) = 2 (z + z^3/3 + ...), z = (m - 1)/(m + 1), |z| < 0.172,
-// 12 odd terms (truncation < 1e-19); ln 2 split in two parts.  Small in
-// registers, for the slow path below (the hot path uses the table).
-DRIFT_HD DRIFT_FI double light_log(double x) {
-    int e;
-    double m = std::frexp(x, &e);  // [0.5, 1)
-    if (m < 0.70710678118654752) {
-        m *= 2.0;
-        e -= 1;
-    }
-    const double z = (m - 1.0) / (m + 1.0);
-    const double z2 = z * z;
-    double p = 1.0 / 23.0;
-#pragma unroll
-    for (int k = 10; k >= 0; --k) p = p * z2 + 1.0 / (2 * k + 1);
-    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
-    return (double)e * ln2_hi + ((double)e * ln2_lo + 2.0 * z * p);
-}
-
-// 2^y: 2^n e^(f ln 2), n = round(y), |f| <= 1/2, Taylor series of e^g to
-// degree 16 (|g| <= 0.35, truncation < 1e-19).
-DRIFT_HD DRIFT_FI double light_exp2(double y) {
-    const double n = std::rint(y);
-    const double g = (y - n) * 0.69314718055994530942;
-    double p = 1.0 / 20922789888000.0;  // 1/16!
-    double fact = 20922789888000.0;
-#pragma unroll
-    for (int k = 15; k >= 0; --k) {
-        fact /= (double)(k + 1);
-        p = p * g + 1.0 / fact;
-    }
-    return std::ldexp(p, (int)n);
-}
-
-// Direct (slow-path) evaluation of S(r2) outside the tabulated range
-// [2^E_MIN, 2^E_MAX), i.e. r < 2.3e-10 or r >= 65536.
-DRIFT_HD DRIFT_FI double spherical_direct(const PotParams& p, double r2) {
-    if (!(r2 > 0.0)) return 0.0;
-    double ir = rsqrt_d(r2);
-    double r = r2 * ir;
-    double ir3 = ir * ir * ir;
-    double sb, sh;
-    if (r2 < 1.0) {
-        // leading terms of the small-radius expansions of both components
-        sb = light_exp2(p.bulge_exponent * light_log(r2) * 1.44269504088896340736) *
-             (p.bulge_small_c0 + p.bulge_small_c1 * r2);
-        double ia = 1.0 / p.halo_a;
-        sh = p.halo_amp * (0.5 * ia * ia * ir - (2.0 / 3.0) * ia * ia * ia + 0.75 * r * ia * ia * ia * ia);
-    } else {
-        // the bulge has its total mass; u = r/a_h is large, no cancellation
-        sb = p.bulge_mass_total * ir3;
-        double u = r / p.halo_a;
-        sh = p.halo_amp * (light_log(1.0 + u) - u / (1.0 + u)) * ir3;
-    }
-    return sb + sh;
+// S(r2) from the far table, for r2 outside the LDS table: any positive
+// finite r2.  At r2 = 0 the spherical force S r_vec vanishes with r_vec, so
+// S(0) is taken as 0; a non-finite r2 gives NaN.
+DRIFT_HD DRIFT_FI double far_spherical(const PotParams& p, double r2) {
+    if (!(r2 > 0.0)) return r2 == 0.0 ? 0.0 : r2;
+    int seg;
+    double s;
+    if (!table_locate_in<FAR_E_MIN, FAR_E_MAX>(r2, seg, s)) return NAN;  // r2 = inf
+    return horner_table(p.far_table + seg * TABLE_NCOEF, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -505,16 +505,31 @@ int resolve_potential(drift_session* s, const drift_potential* potential, PotPar
         TableEntry e;
         e.key = p;
         build_mw_params(p, e.params);
-        std::vector<double> h64(TABLE_DOUBLES);
+        std::vector<double> h64(TABLE_DOUBLES), far(FAR_DOUBLES);
         build_mw_table(p, h64.data());
+        build_mw_far_table(p, far.data());
         std::vector<float> h32(TABLE_DOUBLES);
         for (int k = 0; k < TABLE_DOUBLES; ++k) h32[k] = (float)h64[k];
-        DRIFT_HIP_TRY(hipMalloc(&e.d_table64, TABLE_DOUBLES * sizeof(double)));
-        DRIFT_HIP_TRY(hipMalloc(&e.d_table32, TABLE_DOUBLES * sizeof(float)));
-        DRIFT_HIP_TRY(hipMemcpy(e.d_table64, h64.data(), TABLE_DOUBLES * sizeof(double),
-                                hipMemcpyHostToDevice));
-        DRIFT_HIP_TRY(hipMemcpy(e.d_table32, h32.data(), TABLE_DOUBLES * sizeof(float),
-                                hipMemcpyHostToDevice));
+        hipError_t err = hipMalloc(&e.d_table64, TABLE_DOUBLES * sizeof(double));
+        if (err == hipSuccess) err = hipMalloc(&e.d_table32, TABLE_DOUBLES * sizeof(float));
+        if (err == hipSuccess) err = hipMalloc(&e.d_far, FAR_DOUBLES * sizeof(double));
+        if (err == hipSuccess)
+            err = hipMemcpy(e.d_table64, h64.data(), TABLE_DOUBLES * sizeof(double),
+                            hipMemcpyHostToDevice);
+        if (err == hipSuccess)
+            err = hipMemcpy(e.d_table32, h32.data(), TABLE_DOUBLES * sizeof(float),
+                            hipMemcpyHostToDevice);
+        if (err == hipSuccess)
+            err = hipMemcpy(e.d_far, far.data(), FAR_DOUBLES * sizeof(double),
+                            hipMemcpyHostToDevice);
+        if (err != hipSuccess) {
+            for (void* q : {(void*)e.d_table64, (void*)e.d_table32, (void*)e.d_far})
+                if (q) (void)hipFree(q);
+            set_last_error(std::string("HIP error: ") + hipGetErrorString(err) +
+                           " uploading the Milky-Way force tables");
+            return err == hipErrorOutOfMemory ? DRIFT_ERR_OUT_OF_MEMORY : DRIFT_ERR_DEVICE;
+        }
+        e.params.far_table = e.d_far;
         s->tables.push_back(e);
         params = e.params;
         *table64 = e.d_table64;
@@ -851,6 +866,7 @@ int drift_session_destroy(drift_session* s) {
     for (auto& t : s->tables) {
         (void)hipFree(t.d_table64);
         (void)hipFree(t.d_table32);
+        (void)hipFree(t.d_far);
     }
     for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats, &s->order_keys,
                             &s->order_slots, &s->order_bins, &s->tune_out, &s->tune_stats})
@@ -1036,9 +1052,12 @@ int drift_mw_table_check(const drift_potential* potential, int32_t samples_per_s
     if (potential->kind != POT_MW) return fail(DRIFT_ERR_INVALID_ARGUMENT, "potential is not MW");
     int rc = validate_potential(*potential);
     if (rc != DRIFT_OK) return rc;
-    std::vector<double> coeffs(TABLE_DOUBLES);
+    std::vector<double> coeffs(TABLE_DOUBLES), far(FAR_DOUBLES);
     build_mw_table(*potential, coeffs.data());
-    *max_rel_error = mw_table_max_rel_error(*potential, coeffs.data(), samples_per_segment);
+    build_mw_far_table(*potential, far.data());
+    *max_rel_error = std::max(
+        mw_table_max_rel_error(*potential, coeffs.data(), samples_per_segment),
+        mw_far_table_max_rel_error(*potential, far.data(), samples_per_segment));
     return DRIFT_OK;
 }
 

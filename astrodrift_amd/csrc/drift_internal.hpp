@@ -17,6 +17,7 @@ struct TableEntry {
     PotParams params;
     double* d_table64 = nullptr;
     float* d_table32 = nullptr;
+    double* d_far = nullptr;  // FAR_DOUBLES, global memory
 };
 
 struct DeviceBuffer {
@@ -84,6 +85,9 @@ int resolve_potential(drift_session* s, const drift_potential* potential, PotPar
 void build_mw_params(const drift_potential& p, PotParams& out);
 void build_mw_table(const drift_potential& p, double* coeffs);
 double mw_table_max_rel_error(const drift_potential& p, const double* coeffs, int samples_per_seg);
+void build_mw_far_table(const drift_potential& p, double* coeffs);
+double mw_far_table_max_rel_error(const drift_potential& p, const double* coeffs,
+                                  int samples_per_seg);
 double mw_spherical_exact(const drift_potential& p, double r2);
 
 // Thread-local error message (drift_last_error).

@@ -60,7 +60,7 @@ struct PlummerForce {
 // at TABLE_DOUBLES coefficients (in LDS inside the kernels).
 template <class FT>
 struct MwForce {
-    const PotParams* p;  // slow-path parameters (global/constant memory)
+    const PotParams* p;  // far-table pointer (outside the LDS table's range)
     const FT* table;
     FT disk_amp, disk_a, disk_b2;
     DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
@@ -75,7 +75,7 @@ struct MwForce {
         if (table_locate(r2, seg, s)) {
             S = horner_table<FT>(table + seg * TABLE_NCOEF, FT(s));
         } else {
-            S = FT(spherical_direct(*p, r2));
+            S = FT(far_spherical(*p, r2));
         }
         a[0] = q[3];
         a[1] = q[4];

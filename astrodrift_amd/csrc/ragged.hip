@@ -36,13 +36,13 @@ struct LaneForce {
     int32_t kind;
     FT c0, c1, c2;
     const FT* table;
-    const PotParams* slow;
+    const PotParams* p;  // far table (MW outside the LDS range)
     __device__ __forceinline__ void operator()(const double* q, double* a) const {
         switch (kind) {
             case POT_GALPY_KEPLER: GalpyKeplerForce<FT>{}(q, a); break;
             case POT_KEPLER: KeplerForce<FT>{c0}(q, a); break;
             case POT_PLUMMER: PlummerForce<FT>{c0, c1}(q, a); break;
-            default: MwForce<FT>{slow, table, c0, c1, c2}(q, a); break;
+            default: MwForce<FT>{p, table, c0, c1, c2}(q, a); break;
         }
     }
 };
@@ -52,7 +52,7 @@ __device__ __forceinline__ LaneForce<FT> lane_force(const RaggedArgs& a, int32_t
     const PotParams* p = a.pots + pot;
     LaneForce<FT> f;
     f.kind = p->kind;
-    f.slow = p;
+    f.p = p;
     if (p->kind == POT_MW) {
         f.c0 = FT(p->disk_amp);
         f.c1 = FT(p->disk_a);

@@ -94,59 +94,37 @@ MwSpherical make_spherical(const drift_potential& p) {
     return f;
 }
 
-void segment_bounds(int seg, ld& lo, ld& hi, int& e) {
-    e = seg / 2 + TABLE_E_MIN;
+void segment_bounds(int seg, int e_min, ld& lo, ld& hi, int& e) {
+    e = seg / 2 + e_min;
     const double kSqrt2 = 1.4142135623730951;
     const int half = seg % 2;
     lo = half ? (ld)kSqrt2 : 1.0L;
     hi = half ? 2.0L : (ld)kSqrt2;
 }
 
-}  // namespace
-
-void build_mw_params(const drift_potential& p, PotParams& out) {
-    out = PotParams{};
-    out.kind = POT_MW;
-    out.has_table = 1;
-    out.disk_amp = p.disk_amp;
-    out.disk_a = p.disk_a;
-    out.disk_b2 = p.disk_b * p.disk_b;
-    const long double a = 1.5L - 0.5L * (long double)p.bulge_alpha;
-    const long double mass_scale = 2 * 3.14159265358979323846264338327950288L *
-                                   (long double)p.bulge_amp *
-                                   powl((long double)p.bulge_rc, 3 - (long double)p.bulge_alpha);
-    const long double rc2 = (long double)p.bulge_rc * (long double)p.bulge_rc;
-    out.bulge_mass_total = (double)(mass_scale * tgammal(a));
-    // gamma(a, x) = x^a (1/a - x/(a+1) + ...), x = r2/rc2:
-    // S_b = mass_scale rc2^-a r2^(a - 3/2) (1/a - r2/(rc2 (a+1))).
-    const long double pre = mass_scale * powl(rc2, -a);
-    out.bulge_small_c0 = (double)(pre / a);
-    out.bulge_small_c1 = (double)(-pre / (rc2 * (a + 1)));
-    out.bulge_exponent = (double)(a - 1.5L);
-    out.halo_amp = p.halo_amp;
-    out.halo_a = p.halo_a;
-}
-
-void build_mw_table(const drift_potential& p, double* coeffs) {
+// Chebyshev interpolants of S on the half-octave segments of [2^e_min,
+// 2^e_max), as monomial coefficients in the segment's local variable.
+void build_table(const drift_potential& p, int e_min, int e_max, double* coeffs) {
     const MwSpherical f = make_spherical(p);
     const int N = TABLE_NCOEF;
     const ld pi = 3.14159265358979323846264338327950288L;
-    for (int seg = 0; seg < TABLE_SEGMENTS; ++seg) {
+    ld node[TABLE_NCOEF], basis[TABLE_NCOEF][TABLE_NCOEF];
+    for (int j = 0; j < N; ++j) node[j] = cosl(pi * (j + 0.5L) / N);
+    for (int k = 0; k < N; ++k)
+        for (int j = 0; j < N; ++j) basis[k][j] = cosl(k * pi * (j + 0.5L) / N);
+    const int segments = 2 * (e_max - e_min);
+    for (int seg = 0; seg < segments; ++seg) {
         ld lo, hi;
         int e;
-        segment_bounds(seg, lo, hi, e);
+        segment_bounds(seg, e_min, lo, hi, e);
         const ld scale = ldexpl(1.0L, e);
         const ld centre = 0.5L * (lo + hi), half = 0.5L * (hi - lo);
         // Chebyshev interpolation at the N first-kind nodes.
         ld fv[TABLE_NCOEF], cheb[TABLE_NCOEF];
-        for (int j = 0; j < N; ++j) {
-            ld th = pi * (j + 0.5L) / N;
-            ld s = cosl(th);
-            fv[j] = f((centre + half * s) * scale);
-        }
+        for (int j = 0; j < N; ++j) fv[j] = f((centre + half * node[j]) * scale);
         for (int k = 0; k < N; ++k) {
             ld acc = 0;
-            for (int j = 0; j < N; ++j) acc += fv[j] * cosl(k * pi * (j + 0.5L) / N);
+            for (int j = 0; j < N; ++j) acc += fv[j] * basis[k][j];
             cheb[k] = 2 * acc / N;
         }
         cheb[0] *= 0.5L;
@@ -169,29 +147,69 @@ void build_mw_table(const drift_potential& p, double* coeffs) {
     }
 }
 
-double mw_table_max_rel_error(const drift_potential& p, const double* coeffs, int samples_per_seg) {
+}  // namespace
+
+void build_mw_params(const drift_potential& p, PotParams& out) {
+    out = PotParams{};
+    out.kind = POT_MW;
+    out.has_table = 1;
+    out.disk_amp = p.disk_amp;
+    out.disk_a = p.disk_a;
+    out.disk_b2 = p.disk_b * p.disk_b;
+    out.far_table = nullptr;  // set when the far table is on the device
+}
+
+void build_mw_table(const drift_potential& p, double* coeffs) {
+    build_table(p, TABLE_E_MIN, TABLE_E_MAX, coeffs);
+}
+
+void build_mw_far_table(const drift_potential& p, double* coeffs) {
+    build_table(p, FAR_E_MIN, FAR_E_MAX, coeffs);
+}
+
+// Worst relative error of a table of [2^e_min, 2^e_max) against the long-double
+// S at samples_per_seg points per segment, over the segments where S and its
+// coefficients are normal doubles (|S| in [2^-960, 2^1000]).
+double table_max_rel_error(const drift_potential& p, const double* coeffs, int e_min, int e_max,
+                           int samples_per_seg) {
     const MwSpherical f = make_spherical(p);
     double worst = 0.0;
     uint64_t state = 0x9E3779B97F4A7C15ull;
-    for (int seg = 0; seg < TABLE_SEGMENTS; ++seg) {
+    const int segments = 2 * (e_max - e_min);
+    for (int seg = 0; seg < segments; ++seg) {
         ld lo, hi;
         int e;
-        segment_bounds(seg, lo, hi, e);
+        segment_bounds(seg, e_min, lo, hi, e);
         for (int k = 0; k < samples_per_seg; ++k) {
             state = state * 6364136223846793005ull + 1442695040888963407ull;
             double u = (double)(state >> 11) * (1.0 / 9007199254740992.0);
             double m = (double)(lo + (hi - lo) * (ld)u);
             double r2 = std::ldexp(m, e);
-            int sg;
-            double s;
-            if (!table_locate(r2, sg, s)) continue;
-            double approx = horner_table(coeffs + sg * TABLE_NCOEF, s);
             ld exact = f((ld)r2);
+            if (!(fabsl(exact) >= ldexpl(1.0L, -960) && fabsl(exact) <= ldexpl(1.0L, 1000))) continue;
+            // the local coordinate as the device computes it
+            int dm;
+            double mm = std::frexp(r2, &dm) * 2.0;
+            const double kSqrt2 = 1.4142135623730951;
+            const int half = mm >= kSqrt2 ? 1 : 0;
+            const double lo_d = half ? kSqrt2 : 1.0, hi_d = half ? 2.0 : kSqrt2;
+            const double s = (mm - 0.5 * (lo_d + hi_d)) * (2.0 / (hi_d - lo_d));
+            const int sg = 2 * ((dm - 1) - e_min) + half;
+            double approx = horner_table(coeffs + sg * TABLE_NCOEF, s);
             double rel = (double)fabsl(((ld)approx - exact) / exact);
             if (rel > worst) worst = rel;
         }
     }
     return worst;
+}
+
+double mw_table_max_rel_error(const drift_potential& p, const double* coeffs, int samples_per_seg) {
+    return table_max_rel_error(p, coeffs, TABLE_E_MIN, TABLE_E_MAX, samples_per_seg);
+}
+
+double mw_far_table_max_rel_error(const drift_potential& p, const double* coeffs,
+                                  int samples_per_seg) {
+    return table_max_rel_error(p, coeffs, FAR_E_MIN, FAR_E_MAX, samples_per_seg);
 }
 
 double mw_spherical_exact(const drift_potential& p, double r2) {

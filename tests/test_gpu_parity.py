@@ -231,8 +231,12 @@ def test_force_parity(gpu, precision, pot):
     spec = specs[pot]
     rng = np.random.default_rng(3)
     pos = rng.uniform(-3, 3, (4096, 3))
-    pos[:16] *= 1e-4  # near the centre (MW slow path below the table)
-    pos[16:32] *= 40.0  # far out (MW slow path above the table)
+    pos[:16] *= 1e-4  # near the centre
+    pos[16:32] *= 40.0  # far out
+    # outside the MW LDS table (r < 2.3e-10 or r >= 65536): the far table
+    pos[32:40] *= np.array([1e-11, 3e-12, 1e-12, 5e-11, 2e5, 1e6, 5e6, 1e7])[:, None]
+    if precision == _native.PRECISION_FP64 and pot == "mw":
+        pos[40:44] *= np.array([1e-100, 1e-150, 1e50, 1e100])[:, None]
     dp = torch.from_numpy(pos).cuda()
     da = torch.zeros_like(dp)
     _native.force_device(spec, dp.data_ptr(), da.data_ptr(), pos.shape[0], precision)
@@ -241,10 +245,14 @@ def test_force_parity(gpu, precision, pot):
     ref = np.array([oracle_lib.force(spec, *p) for p in pos])
     if pot == "mw":
         # Near the centre the restated NFW formula ln(1+u) - u/(1+u) cancels
-        # (relative error ~ 1e-16/u); there the reference is an independent
-        # long-double + scipy evaluation of the same closed forms.
+        # (relative error ~ 1e-16/u); there, and at the extreme radii, the
+        # reference is an independent long-double + scipy evaluation of the
+        # same closed forms.
         ref[:16] = [mw_force_independent(spec, *p) for p in pos[:16]]
-    rel = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        ref[32:44] = [mw_force_independent(spec, *p) for p in pos[32:44]]
+    # per-row scaling keeps the norms of the 1e-198-sized forces from underflowing
+    scale = np.max(np.abs(ref), axis=1, keepdims=True)
+    rel = np.linalg.norm((acc - ref) / scale, axis=1) / np.linalg.norm(ref / scale, axis=1)
     bound = 1e-13 if precision == _native.PRECISION_FP64 else 2e-6
     print(f"{pot} precision={precision}: max rel force error {rel.max():.3e}")
     assert rel.max() < bound
