@@ -194,8 +194,12 @@ __device__ __forceinline__ bool claim_item(bool need, unsigned long long* queue,
     return slot < items;
 }
 
+// Two wavefronts per SIMD, as for the single-job kernels: the register budget
+// drops to 256 and the latency of one wave's force loads hides behind the other.
+#define DRIFT_RAGGED_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
+
 template <class FT>
-__global__ void __launch_bounds__(256) ragged_dop853_kernel(RaggedArgs a) {
+__global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dop853_kernel(RaggedArgs a) {
     Dop853Lane L;
     Dop853Job J;
     LaneForce<FT> f;
@@ -253,7 +257,7 @@ __global__ void __launch_bounds__(256) ragged_dop853_kernel(RaggedArgs a) {
 }
 
 template <class FT>
-__global__ void __launch_bounds__(256) ragged_dopr54_kernel(RaggedArgs a) {
+__global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dopr54_kernel(RaggedArgs a) {
     Dopr54Lane L;
     Dopr54Job J;
     LaneForce<FT> f;
