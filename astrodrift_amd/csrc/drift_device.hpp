@@ -32,24 +32,28 @@ enum : int32_t {
 // Spherical-part table of the Milky-Way model (bulge + NFW halo):
 //   a_sph(r) = -S(r^2) * r_vec,
 // S(r2) = M_bulge(r)/r^3 + amp_h (ln(1 + r/a_h) - r/(a_h + r)) / r^3.
-// S is tabulated on half-octave segments of r2 in [2^E_MIN, 2^E_MAX): segment
-// index from the binary exponent of r2 and whether its mantissa is >= sqrt(2),
-// TABLE_DEG+1 monomial coefficients per segment in the local variable
-// s = (m - centre) * inv_half in [-1, 1], m = r2 / 2^e in [1, 2).
-// The LDS table covers [2^-64, 2^32) (r from 2.3e-10 to 65,536).  A second,
-// "far" table of the same layout covers every positive finite double r2,
-// [2^-1074, 2^1024), and stays in global memory: it is read only when an
-// orbit leaves the LDS range, with the same few registers as the LDS path.
-constexpr int TABLE_E_MIN = -64;
+// S is tabulated on 2^TABLE_SUBBITS segments per octave of r2, found from the
+// bits of r2 alone: the biased exponent and the top TABLE_SUBBITS mantissa
+// bits form the segment key, and the remaining mantissa bits are the position
+// t in [0, 1) inside the segment.  Each segment holds TABLE_NCOEF monomial
+// coefficients of a Chebyshev interpolant in the local variable
+// s = 2t - 1 in [-1, 1].  The LDS table covers r2 in [2^TABLE_E_MIN,
+// 2^TABLE_E_MAX) (r from 9.5e-7 to 65,536).  A second, "far" table of the
+// same layout covers every positive finite double r2, [2^-1074, 2^1024)
+// (subnormal r2 are scaled by 2^64 to find their key); it stays in global
+// memory and is read only when an orbit leaves the LDS range.
+constexpr int TABLE_SUBBITS = 3;
+constexpr int TABLE_E_MIN = -40;
 constexpr int TABLE_E_MAX = 32;
-constexpr int TABLE_DEG = 15;
+constexpr int TABLE_DEG = 11;
 constexpr int TABLE_NCOEF = TABLE_DEG + 1;
-constexpr int TABLE_SEGMENTS = 2 * (TABLE_E_MAX - TABLE_E_MIN);
+constexpr int TABLE_SEGMENTS = (TABLE_E_MAX - TABLE_E_MIN) << TABLE_SUBBITS;
 constexpr int TABLE_DOUBLES = TABLE_SEGMENTS * TABLE_NCOEF;
 constexpr int FAR_E_MIN = -1074;
 constexpr int FAR_E_MAX = 1024;
-constexpr int FAR_SEGMENTS = 2 * (FAR_E_MAX - FAR_E_MIN);
+constexpr int FAR_SEGMENTS = (FAR_E_MAX - FAR_E_MIN) << TABLE_SUBBITS;
 constexpr int FAR_DOUBLES = FAR_SEGMENTS * TABLE_NCOEF;
+static_assert((TABLE_NCOEF * 8) % 16 == 0, "segments must stay 16-byte aligned for 128-bit loads");
 
 // Potential parameters in the form the kernels use (derived on the host
 // from drift_potential).
@@ -78,30 +82,55 @@ DRIFT_HD DRIFT_FI float rsqrt_f(float x) {
 #endif
 }
 
-// Segment index and local coordinate of r2 in a table of segments
-// [2^E0, 2^E1).  Returns false when r2 is outside it.
-template <int E0, int E1>
-DRIFT_HD DRIFT_FI bool table_locate_in(double r2, int& seg, double& s) {
-    int e;
-    double m = std::frexp(r2, &e);  // r2 = m 2^e, m in [0.5, 1)
-    m *= 2.0;                        // m in [1, 2)
-    e -= 1;
-    if (!(e >= E0 && e < E1)) return false;
-    const double kSqrt2 = 1.4142135623730951;
-    int half = m >= kSqrt2 ? 1 : 0;
-    // half 0: [1, sqrt2), half 1: [sqrt2, 2)
-    const double lo = half ? kSqrt2 : 1.0;
-    const double hi = half ? 2.0 : kSqrt2;
-    const double centre = 0.5 * (lo + hi);
-    const double inv_half = 2.0 / (hi - lo);
-    s = (m - centre) * inv_half;
-    seg = 2 * (e - E0) + half;
-    return true;
+// Reciprocal square root of a positive finite x: the v_rsq_f64 estimate
+// (about 23 correct bits) refined by one second-order Newton-Raphson step,
+// e = 1 - x y^2, 1/sqrt(x) = y (1 - e)^(-1/2) ~ y (1 + e/2 + 3 e^2 / 8), which
+// leaves an error near one ulp.  Zero and infinite x give NaN (the forces only
+// take it of sums with a positive term, or of a non-finite position that ends
+// the orbit anyway).
+DRIFT_HD DRIFT_FI double rsqrt_pos(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+#else
+    return 1.0 / std::sqrt(x);
+#endif
+}
+
+// Reciprocal of a finite, nonzero, normal x: the v_rcp_f64 estimate refined
+// by two Newton-Raphson steps y <- y + y (1 - x y) (error near one ulp).
+DRIFT_HD DRIFT_FI double rcp_fast(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return fma(y, fma(-x, y, 1.0), y);
+#else
+    return 1.0 / x;
+#endif
+}
+
+// Segment of r2 in a table whose key range starts at biased exponent
+// key0_exp (= e_min + 1023) and holds nseg segments; s is the local
+// coordinate in [-1, 1).  The unsigned difference puts keys below the range,
+// zero, subnormal, negative, infinite and NaN r2 all out of range.
+DRIFT_HD DRIFT_FI bool table_locate_bits(double r2, int32_t key0_exp, uint32_t nseg, int& seg,
+                                         double& s) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, r2);
+    const uint32_t hi = (uint32_t)(b >> 32), lo = (uint32_t)b;
+    const uint32_t rel = (hi >> (20 - TABLE_SUBBITS)) - ((uint32_t)key0_exp << TABLE_SUBBITS);
+    // the mantissa bits below the segment bits, as 1 + t in [1, 2)
+    const uint32_t mhi = (((hi << TABLE_SUBBITS) | (lo >> (32 - TABLE_SUBBITS))) & 0x000FFFFFu) |
+                         0x3FF00000u;
+    const uint64_t mb = ((uint64_t)mhi << 32) | (uint64_t)(lo << TABLE_SUBBITS);
+    s = fma(2.0, __builtin_bit_cast(double, mb), -3.0);  // 2 (1 + t) - 3 = 2t - 1, exact
+    seg = (int)rel;
+    return rel < nseg;
 }
 
 // The LDS table's range (false outside [2^E_MIN, 2^E_MAX) or not finite).
 DRIFT_HD DRIFT_FI bool table_locate(double r2, int& seg, double& s) {
-    return table_locate_in<TABLE_E_MIN, TABLE_E_MAX>(r2, seg, s);
+    return table_locate_bits(r2, TABLE_E_MIN + 1023, (uint32_t)TABLE_SEGMENTS, seg, s);
 }
 
 template <class T>
@@ -117,9 +146,14 @@ DRIFT_HD DRIFT_FI T horner_table(const T* c, T s) {
 // S(0) is taken as 0; a non-finite r2 gives NaN.
 DRIFT_HD DRIFT_FI double far_spherical(const PotParams& p, double r2) {
     if (!(r2 > 0.0)) return r2 == 0.0 ? 0.0 : r2;
+    int32_t key0 = FAR_E_MIN + 1023;
+    if (r2 < 0x1p-1022) {  // subnormal: scale into the normal range, shift the key
+        r2 *= 0x1p64;
+        key0 += 64;
+    }
     int seg;
     double s;
-    if (!table_locate_in<FAR_E_MIN, FAR_E_MAX>(r2, seg, s)) return NAN;  // r2 = inf
+    if (!table_locate_bits(r2, key0, (uint32_t)FAR_SEGMENTS, seg, s)) return NAN;  // r2 = inf
     return horner_table(p.far_table + seg * TABLE_NCOEF, s);
 }
 
@@ -137,7 +171,7 @@ DRIFT_HD DRIFT_FI void galpy_kepler_acc(T x, T y, T z, T& ax, T& ay, T& az) {
     T R2 = x * x + y * y;
     T r2 = R2 + z * z;
     T ir;
-    if constexpr (sizeof(T) == 8) ir = rsqrt_d(r2); else ir = rsqrt_f(r2);
+    if constexpr (sizeof(T) == 8) ir = rsqrt_pos(r2); else ir = rsqrt_f(r2);
     T f = -(ir * ir * ir);
     ax = f * x;
     ay = f * y;
@@ -153,7 +187,7 @@ template <class T>
 DRIFT_HD DRIFT_FI void kepler_acc(T amp, T x, T y, T z, T& ax, T& ay, T& az) {
     T r2 = x * x + y * y + z * z;
     T ir;
-    if constexpr (sizeof(T) == 8) ir = rsqrt_d(r2); else ir = rsqrt_f(r2);
+    if constexpr (sizeof(T) == 8) ir = rsqrt_pos(r2); else ir = rsqrt_f(r2);
     T f = -amp * (ir * ir * ir);
     ax = f * x;
     ay = f * y;
@@ -165,7 +199,7 @@ template <class T>
 DRIFT_HD DRIFT_FI void plummer_acc(T amp, T b2, T x, T y, T z, T& ax, T& ay, T& az) {
     T r2 = x * x + y * y + z * z;
     T ir;
-    if constexpr (sizeof(T) == 8) ir = rsqrt_d(r2 + b2); else ir = rsqrt_f(r2 + b2);
+    if constexpr (sizeof(T) == 8) ir = rsqrt_pos(r2 + b2); else ir = rsqrt_f(r2 + b2);
     T f = -amp * (ir * ir * ir);
     ax = f * x;
     ay = f * y;
@@ -180,10 +214,10 @@ DRIFT_HD DRIFT_FI void mn_acc(T amp, T a, T b2, T x, T y, T z, T& ax, T& ay, T& 
     T R2 = x * x + y * y;
     T zb2 = z * z + b2;
     T izb, iD;
-    if constexpr (sizeof(T) == 8) izb = rsqrt_d(zb2); else izb = rsqrt_f(zb2);
+    if constexpr (sizeof(T) == 8) izb = rsqrt_pos(zb2); else izb = rsqrt_f(zb2);
     T azb = a + zb2 * izb;
     T D2 = azb * azb + R2;
-    if constexpr (sizeof(T) == 8) iD = rsqrt_d(D2); else iD = rsqrt_f(D2);
+    if constexpr (sizeof(T) == 8) iD = rsqrt_pos(D2); else iD = rsqrt_f(D2);
     T f = -amp * (iD * iD * iD);
     ax = f * x;
     ay = f * y;

@@ -15,8 +15,9 @@
 //     the trajectory is unchanged);
 //   * DOP853's err^(1/8) is sqrt(sqrt(sqrt(err))); pow(facold, beta) with
 //     beta = 0 is 1 and is dropped; divisions by a value used more than once
-//     (the error scale sk, 1/safe, the step h in the dense output) become
-//     multiplications by its reciprocal;
+//     (the error scale sk, 1/safe, the step h in the dense output) and the
+//     step-size updates become multiplications by a reciprocal (Newton-refined,
+//     within about an ulp of the quotient);
 //   * DOPR54's log-space error norm s = ln(e^(atol-c) + e^(rtol+max-c)) + c is
 //     the logarithm of sc = e^atol + e^rtol max|y|, so the norm is computed as
 //     mean((yerr/sc)^2), and round(log2(0.85 err^-0.2)) is decided by comparing
@@ -148,6 +149,7 @@ struct Dop853Job {
 struct Dop853Lane {
     double y[DIM], k1[DIM];
     double h, t_cur, t_old;
+    double t_next;  // times[ii + 1] while ii < nt - 1 (the next sample to reach)
     int32_t ii;  // index of the last emitted sample
     int32_t reject;
     uint32_t accepted, rejected, dense;
@@ -204,9 +206,16 @@ DRIFT_HD DRIFT_FI void dop853_init(Dop853Lane& L, const double* y0, double t0, c
     L.t_cur = t0;
     L.t_old = t0;
     L.ii = 0;
+    L.t_next = J.nt > 1 ? J.times[1] : 0.0;
     L.reject = 0;
     L.accepted = L.rejected = L.dense = 0;
     L.status = 0;
+}
+
+// Reloads the cached next sample time after the lane's sample index was
+// restored from saved state.
+DRIFT_HD DRIFT_FI void dop853_refresh_next(Dop853Lane& L, const Dop853Job& J) {
+    L.t_next = L.ii < J.nt - 1 ? J.times[L.ii + 1] : 0.0;
 }
 
 // Writes NaN to the samples not reached and marks the lane finished.
@@ -291,7 +300,7 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
     double err = 0.0, err2 = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
-        const double inv_sk = 1.0 / (J.atol + J.rtol * tmax(std::fabs(y0[i]), std::fabs(k5[i])));
+        const double inv_sk = rcp_fast(J.atol + J.rtol * tmax(std::fabs(y0[i]), std::fabs(k5[i])));
         double erri = k4[i] - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i];
         double sqr = erri * inv_sk;
         err2 += sqr * sqr;
@@ -313,14 +322,14 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
     const double fac11 = std::sqrt(std::sqrt(std::sqrt(err)));  // err^(1/8)
     constexpr double inv_safe = 1.0 / safe;
     const double fac = tmax(facc2, tmin(facc1, fac11 * inv_safe));
-    double hnew = h / fac;
+    double hnew = h * rcp_fast(fac);
 
     if (err <= 1.0) {
         L.accepted++;
         f(k5, k4);  // FSAL derivative at the new point
         const double* times = J.times;
         const int32_t last = J.nt - 1;
-        if (L.ii < last && J.pos_neg * times[L.ii + 1] < J.pos_neg * t_new) {
+        if (L.ii < last && J.pos_neg * L.t_next < J.pos_neg * t_new) {
             // Dense output (dop853.rs:636-763), only when a sample falls in
             // (t_step, t_new).
             L.dense++;
@@ -367,9 +376,14 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
                 r7[i] = h * (r7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
                 r8[i] = h * (r8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
             }
-            const double inv_h = 1.0 / h;
+            const double inv_h = rcp_fast(h);
+            double t_s = L.t_next;
             do {
-                const double sv = (times[L.ii + 1] - t_step) * inv_h;
+                // fetch the following sample time now, so its latency hides
+                // behind this sample's polynomial and stores
+                const int32_t after = L.ii + 2;
+                const double t_after = after <= last ? times[after] : 0.0;
+                const double sv = (t_s - t_step) * inv_h;
                 const double s1 = 1.0 - sv;
                 double yy[DIM];
 #pragma unroll
@@ -381,7 +395,9 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
                 L.ii++;
                 if (!finite6(yy)) L.status |= STATUS_NONFINITE;
                 sink.emit(L.ii, yy);
-            } while (L.ii < last && J.pos_neg * times[L.ii + 1] < J.pos_neg * t_new);
+                t_s = t_after;
+            } while (L.ii < last && J.pos_neg * t_s < J.pos_neg * t_new);
+            L.t_next = t_s;
         }
 #pragma unroll
         for (int i = 0; i < DIM; i++) {
@@ -395,7 +411,7 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
         L.t_cur = t_new;
     } else {
         L.rejected++;
-        hnew = h / tmin(facc1, fac11 * inv_safe);
+        hnew = h * rcp_fast(tmin(facc1, fac11 * inv_safe));
         L.reject = 1;
         // t_current = t_old; t_old = t_old_older (dop853.rs:777-778)
         L.t_cur = t_step;

@@ -228,6 +228,7 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dop853_kernel(R
             stop = d.stop_sample;
             if (phase == kOrbitRunning) {
                 restore(res, L);
+                dop853_refresh_next(L, J);
             } else {
                 double y0[DIM];
                 const double2* p = reinterpret_cast<const double2*>(d.state0 + orbit * DIM);

@@ -5,9 +5,10 @@
 // summed acceleration is -S(r^2) r_vec with
 //   S(r2) = M_b(r) / r^3 + amp_h (ln(1 + r/a_h) - r/(a_h + r)) / r^3,
 //   M_b(r) = 2 pi amp_b rc^(3-alpha) gamma(3/2 - alpha/2, r^2/rc^2).
-// S has its only singularity at r2 = 0, so on every half-octave of r2 it is
-// approximated to ~1e-16 by a Chebyshev interpolant, evaluated in long double
-// here and stored as monomial coefficients in the segment's local variable.
+// S has its only singularity at r2 = 0, so on every segment of r2 (2^K equal
+// mantissa intervals per octave, K = TABLE_SUBBITS) it is approximated to
+// ~1e-16 by a Chebyshev interpolant, evaluated in long double here and stored
+// as monomial coefficients in the segment's local variable.
 // The kernels then need no log, exp or pow for these two components.
 //
 // gamma(a, x) is evaluated by its power series for x < a + 1 and by the Lentz
@@ -94,16 +95,18 @@ MwSpherical make_spherical(const drift_potential& p) {
     return f;
 }
 
+// Octave e and the interval [lo, hi) of the mantissa m = r2 / 2^e covered by
+// segment `seg` of a table that starts at octave e_min.
 void segment_bounds(int seg, int e_min, ld& lo, ld& hi, int& e) {
-    e = seg / 2 + e_min;
-    const double kSqrt2 = 1.4142135623730951;
-    const int half = seg % 2;
-    lo = half ? (ld)kSqrt2 : 1.0L;
-    hi = half ? 2.0L : (ld)kSqrt2;
+    const int sub = seg & ((1 << TABLE_SUBBITS) - 1);
+    e = (seg >> TABLE_SUBBITS) + e_min;
+    const ld w = ldexpl(1.0L, -TABLE_SUBBITS);
+    lo = 1.0L + sub * w;
+    hi = lo + w;
 }
 
-// Chebyshev interpolants of S on the half-octave segments of [2^e_min,
-// 2^e_max), as monomial coefficients in the segment's local variable.
+// Chebyshev interpolants of S on the segments of [2^e_min, 2^e_max), as
+// monomial coefficients in the segment's local variable s in [-1, 1].
 void build_table(const drift_potential& p, int e_min, int e_max, double* coeffs) {
     const MwSpherical f = make_spherical(p);
     const int N = TABLE_NCOEF;
@@ -112,7 +115,7 @@ void build_table(const drift_potential& p, int e_min, int e_max, double* coeffs)
     for (int j = 0; j < N; ++j) node[j] = cosl(pi * (j + 0.5L) / N);
     for (int k = 0; k < N; ++k)
         for (int j = 0; j < N; ++j) basis[k][j] = cosl(k * pi * (j + 0.5L) / N);
-    const int segments = 2 * (e_max - e_min);
+    const int segments = (e_max - e_min) << TABLE_SUBBITS;
     for (int seg = 0; seg < segments; ++seg) {
         ld lo, hi;
         int e;
@@ -169,13 +172,15 @@ void build_mw_far_table(const drift_potential& p, double* coeffs) {
 
 // Worst relative error of a table of [2^e_min, 2^e_max) against the long-double
 // S at samples_per_seg points per segment, over the segments where S and its
-// coefficients are normal doubles (|S| in [2^-960, 2^1000]).
+// coefficients are normal doubles (|S| in [2^-960, 2^1000]).  Each point is
+// located the way the kernels locate it (table_locate_bits, and for the far
+// table the subnormal rescaling of far_spherical).
 double table_max_rel_error(const drift_potential& p, const double* coeffs, int e_min, int e_max,
                            int samples_per_seg) {
     const MwSpherical f = make_spherical(p);
     double worst = 0.0;
     uint64_t state = 0x9E3779B97F4A7C15ull;
-    const int segments = 2 * (e_max - e_min);
+    const int segments = (e_max - e_min) << TABLE_SUBBITS;
     for (int seg = 0; seg < segments; ++seg) {
         ld lo, hi;
         int e;
@@ -184,17 +189,21 @@ double table_max_rel_error(const drift_potential& p, const double* coeffs, int e
             state = state * 6364136223846793005ull + 1442695040888963407ull;
             double u = (double)(state >> 11) * (1.0 / 9007199254740992.0);
             double m = (double)(lo + (hi - lo) * (ld)u);
+            if (m >= (double)hi) continue;
             double r2 = std::ldexp(m, e);
             ld exact = f((ld)r2);
             if (!(fabsl(exact) >= ldexpl(1.0L, -960) && fabsl(exact) <= ldexpl(1.0L, 1000))) continue;
-            // the local coordinate as the device computes it
-            int dm;
-            double mm = std::frexp(r2, &dm) * 2.0;
-            const double kSqrt2 = 1.4142135623730951;
-            const int half = mm >= kSqrt2 ? 1 : 0;
-            const double lo_d = half ? kSqrt2 : 1.0, hi_d = half ? 2.0 : kSqrt2;
-            const double s = (mm - 0.5 * (lo_d + hi_d)) * (2.0 / (hi_d - lo_d));
-            const int sg = 2 * ((dm - 1) - e_min) + half;
+            int32_t key0 = e_min + 1023;
+            double r2k = r2;
+            if (r2k < 0x1p-1022) {
+                r2k *= 0x1p64;
+                key0 += 64;
+            }
+            int sg;
+            double s;
+            // (subnormal r2 may round into a neighbouring segment: the point is
+            // then checked where the kernels would look it up)
+            if (!table_locate_bits(r2k, key0, (uint32_t)segments, sg, s)) return INFINITY;
             double approx = horner_table(coeffs + sg * TABLE_NCOEF, s);
             double rel = (double)fabsl(((ld)approx - exact) / exact);
             if (rel > worst) worst = rel;
