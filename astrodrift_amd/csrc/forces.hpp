@@ -64,11 +64,21 @@ struct MwForce {
     const FT* table;
     FT disk_amp, disk_a, disk_b2;
     DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
-        FT x = FT(q[0]), y = FT(q[1]), z = FT(q[2]);
-        FT dx, dy, dz;
-        mn_acc<FT>(disk_amp, disk_a, disk_b2, x, y, z, dx, dy, dz);
-        // spherical part, table lookup in double coordinates of r2
-        double r2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+        const FT x = FT(q[0]), y = FT(q[1]), z = FT(q[2]);
+        // Miyamoto-Nagai disk (shared/src/potential.rs:138-151), in the
+        // factored form a_R = f R, a_z = f z (a + zb) / zb with
+        // f = -amp / D^3, D^2 = R^2 + (a + zb)^2, zb = sqrt(z^2 + b^2)
+        const FT R2 = x * x + y * y;
+        const FT zb2 = std::fma(z, z, disk_b2);
+        FT izb, iD;
+        if constexpr (sizeof(FT) == 8) izb = rsqrt_pos(zb2); else izb = rsqrt_f(zb2);
+        const FT azb = disk_a + zb2 * izb;
+        const FT D2 = azb * azb + R2;
+        if constexpr (sizeof(FT) == 8) iD = rsqrt_pos(D2); else iD = rsqrt_f(D2);
+        const FT f = -disk_amp * (iD * iD * iD);
+        // spherical part (bulge + halo), -S r_vec, table lookup on r2 in double
+        double r2;
+        if constexpr (sizeof(FT) == 8) r2 = z * z + R2; else r2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
         int seg;
         double s;
         FT S;
@@ -77,12 +87,14 @@ struct MwForce {
         } else {
             S = FT(far_spherical(*p, r2));
         }
+        // both parts are multiples of (x, y) in the plane: a = (f - S) (x, y)
+        const FT g = f - S;
         a[0] = q[3];
         a[1] = q[4];
         a[2] = q[5];
-        a[3] = double(dx - S * x);
-        a[4] = double(dy - S * y);
-        a[5] = double(dz - S * z);
+        a[3] = double(g * x);
+        a[4] = double(g * y);
+        a[5] = double((f * (azb * izb) - S) * z);
     }
 };
 

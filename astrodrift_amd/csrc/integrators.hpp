@@ -17,7 +17,8 @@
 //     beta = 0 is 1 and is dropped; divisions by a value used more than once
 //     (the error scale sk, 1/safe, the step h in the dense output) and the
 //     step-size updates become multiplications by a reciprocal (Newton-refined,
-//     within about an ulp of the quotient);
+//     within about an ulp of the quotient); the dense coefficients rcont5..8
+//     keep their common factor h out, and each sample applies it once;
 //   * DOPR54's log-space error norm s = ln(e^(atol-c) + e^(rtol+max-c)) + c is
 //     the logarithm of sc = e^atol + e^rtol max|y|, so the norm is computed as
 //     mean((yerr/sc)^2), and round(log2(0.85 err^-0.2)) is decided by comparing
@@ -369,12 +370,14 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
                                       a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] +
                                       a1615 * k2[i]);
             f(yy1, k3);
+            // rcont5..8 without their common factor h (dop853.rs:743-746);
+            // h is applied once per sample, as s1 * h, below
 #pragma unroll
             for (int i = 0; i < DIM; i++) {
-                r5[i] = h * (r5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
-                r6[i] = h * (r6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
-                r7[i] = h * (r7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
-                r8[i] = h * (r8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
+                r5[i] = r5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i];
+                r6[i] = r6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i];
+                r7[i] = r7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i];
+                r8[i] = r8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i];
             }
             const double inv_h = rcp_fast(h);
             double t_s = L.t_next;
@@ -385,13 +388,14 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
                 const double t_after = after <= last ? times[after] : 0.0;
                 const double sv = (t_s - t_step) * inv_h;
                 const double s1 = 1.0 - sv;
+                const double s1h = s1 * h;
                 double yy[DIM];
 #pragma unroll
                 for (int i = 0; i < DIM; i++)
                     yy[i] = r1[i] +
                             sv * (r2[i] +
                                   s1 * (r3[i] +
-                                        sv * (r4[i] + s1 * (r5[i] + sv * (r6[i] + s1 * (r7[i] + sv * r8[i]))))));
+                                        sv * (r4[i] + s1h * (r5[i] + sv * (r6[i] + s1 * (r7[i] + sv * r8[i]))))));
                 L.ii++;
                 if (!finite6(yy)) L.status |= STATUS_NONFINITE;
                 sink.emit(L.ii, yy);
