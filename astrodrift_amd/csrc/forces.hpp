@@ -81,12 +81,14 @@ struct MwForce {
         if constexpr (sizeof(FT) == 8) r2 = z * z + R2; else r2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
         int seg;
         double s;
-        FT S;
-        if (table_locate(r2, seg, s)) {
-            S = horner_table<FT>(table + seg * TABLE_NCOEF, FT(s));
-        } else {
-            S = FT(far_spherical(*p, r2));
-        }
+        const bool in_lds = table_locate(r2, seg, s);
+        // every lane evaluates the LDS table (a lane outside its range reads
+        // the last segment), and only a lane outside it branches to the far
+        // table, so the common path carries no divergent branch
+        const uint32_t last_row = TABLE_SEGMENTS - 1;
+        const uint32_t row = (uint32_t)seg < last_row ? (uint32_t)seg : last_row;
+        FT S = horner_table<FT>(table + row * TABLE_NCOEF, FT(s));
+        if (!in_lds) S = FT(far_spherical(*p, r2));
         // both parts are multiples of (x, y) in the plane: a = (f - S) (x, y)
         const FT g = f - S;
         a[0] = q[3];

@@ -10,6 +10,12 @@
 //   DOP853: src/integrators/galpy/dop853.rs:316-783 (galpy leung_dop853.c)
 //   DOPR54: src/integrators/galpy/dopr54.rs:135-778 (galpy bovy_rk.c)
 // with these GPU-specific, value-preserving restatements:
+//   * DOP853 runs in its second-order (Runge-Kutta-Nystrom) form: the
+//     right-hand side is (v, F(x)), so the velocity half of each stage
+//     derivative is the stage velocity, a known combination of earlier forces;
+//     every combination of stage derivatives is formed from the forces alone
+//     with exact products of the tableau (dop853_rkn.hpp), equal to the
+//     reference's up to rounding, and the stage velocities are never formed;
 //   * DOP853 evaluates its three dense-output stages only on accepted steps
 //     that emit at least one sample (k10/k2/k3 are recomputed before reuse, so
 //     the trajectory is unchanged);
@@ -29,6 +35,7 @@
 #pragma once
 
 #include "drift_device.hpp"
+#include "dop853_rkn.hpp"
 
 namespace drift {
 
@@ -138,8 +145,38 @@ DRIFT_HD DRIFT_FI bool finite6(const double* y) {
     return ok;
 }
 
+// The DOP853 coefficient table (dop853_rkn.hpp).  On the device it is read
+// from constant memory through a pointer that each use site re-derives behind
+// an empty asm statement that also takes a value the site depends on (`after`,
+// e.g. the previous stage's force), so the compiler fetches a site's
+// coefficients with 64-byte scalar loads next to their use instead of hoisting
+// all ~200 of them out of the step (there are not enough scalar registers to
+// keep them) or materialising every 64-bit literal with two scalar moves.
+#if defined(__HIP_DEVICE_COMPILE__)
+using Dop853Coef = const __attribute__((address_space(4))) double*;
+__device__ __forceinline__ Dop853Coef dop853_coef_base() {
+    return (Dop853Coef)rkn853::kCoefDevice;
+}
+__device__ __forceinline__ Dop853Coef dop853_coef(Dop853Coef base, double after) {
+    asm volatile("" : "+s"(base) : "v"(after));
+    return base;
+}
+// the same, after all six components of a group that came before
+__device__ __forceinline__ Dop853Coef dop853_coef(Dop853Coef base, const double* after) {
+    asm volatile("" : "+s"(base) : "v"(after[0]), "v"(after[1]), "v"(after[2]), "v"(after[3]),
+                 "v"(after[4]), "v"(after[5]));
+    return base;
+}
+#else
+using Dop853Coef = const double*;
+inline Dop853Coef dop853_coef_base() { return rkn853::kCoef; }
+inline Dop853Coef dop853_coef(Dop853Coef base, double) { return base; }
+inline Dop853Coef dop853_coef(Dop853Coef base, const double*) { return base; }
+#endif
+
 // Uniform (per-job) DOP853 data.
 struct Dop853Job {
+    Dop853Coef coef;  // the coefficient table (dop853_rkn.hpp)
     const double* times;
     int32_t nt;
     uint32_t max_attempts;
@@ -148,7 +185,8 @@ struct Dop853Job {
 };
 
 struct Dop853Lane {
-    double y[DIM], k1[DIM];
+    double y[DIM];   // (x, v)
+    double F1[3];    // force at y: the acceleration half of the first stage derivative
     double h, t_cur, t_old;
     double t_next;  // times[ii + 1] while ii < nt - 1 (the next sample to reach)
     int32_t ii;  // index of the last emitted sample
@@ -161,6 +199,7 @@ DRIFT_HD DRIFT_FI Dop853Job make_dop853_job(const double* times, int32_t nt, dou
                                              double atol_lin, uint32_t max_attempts, double t0,
                                              double tend) {
     Dop853Job j;
+    j.coef = dop853_coef_base();
     j.times = times;
     j.nt = nt;
     j.max_attempts = max_attempts;
@@ -178,12 +217,15 @@ DRIFT_HD DRIFT_FI void dop853_init(Dop853Lane& L, const double* y0, double t0, c
 #pragma unroll
     for (int i = 0; i < DIM; i++) L.y[i] = y0[i];
     sink.emit(0, L.y);
-    f(L.y, L.k1);
+    double k1[DIM];
+    f(L.y, k1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) L.F1[c] = k1[3 + c];
     double dnf = 0.0, dny = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
         double sk = J.atol + J.rtol * std::fabs(L.y[i]);
-        double sqr = L.k1[i] / sk;
+        double sqr = k1[i] / sk;
         dnf += sqr * sqr;
         sqr = L.y[i] / sk;
         dny += sqr * sqr;
@@ -191,13 +233,13 @@ DRIFT_HD DRIFT_FI void dop853_init(Dop853Lane& L, const double* y0, double t0, c
     double h = sign_of(tmin(std::sqrt(dny / dnf) * 0.01, std::fabs(J.hmax)), J.pos_neg);
     double k3[DIM], k2[DIM];
 #pragma unroll
-    for (int i = 0; i < DIM; i++) k3[i] = L.y[i] + h * L.k1[i];
+    for (int i = 0; i < DIM; i++) k3[i] = L.y[i] + h * k1[i];
     f(k3, k2);
     double der2 = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
         double sk = J.atol + J.rtol * std::fabs(L.y[i]);
-        double sqr = (k2[i] - L.k1[i]) / sk;
+        double sqr = (k2[i] - k1[i]) / sk;
         der2 += sqr * sqr;
     }
     der2 = std::sqrt(der2) / h;
@@ -233,81 +275,102 @@ DRIFT_HD DRIFT_FI void abandon_orbit(int32_t& ii, int32_t nt, int32_t& status, i
     status |= flag;
 }
 
+// Evaluates stage I of the Runge-Kutta-Nystrom form: the stage position
+// X_I = x0 + h c_I v0 + h^2 sum_m (A^2)_Im F_m (the stage velocity, which the
+// force does not read, is never formed) and its force F_I.
+template <int I, class F>
+DRIFT_HD DRIFT_FI void rkn853_stage(const Dop853Job& J, const double* y0, double h, double hh,
+                                    double (&Fs)[17][3], const F& f) {
+    const Dop853Coef C = dop853_coef(J.coef, Fs[I - 1][2]);
+    const double hc = h * C[rkn853::kStageC[I]];
+    double q[DIM], a[DIM];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if constexpr (I == 2)
+            q[c] = y0[c] + hc * y0[3 + c];
+        else
+            q[c] = y0[c] + (hc * y0[3 + c] + hh * rkn853::stage_sum<I>(Fs, c, C));
+        q[3 + c] = 0.0;
+    }
+    f(q, a);
+#pragma unroll
+    for (int c = 0; c < 3; c++) Fs[I][c] = a[3 + c];
+}
+
 // One attempted DOP853 step, dop853.rs:425-782.  Returns true when the
 // orbit has emitted its last sample.
+//
+// The right-hand side is (v, F(x)), so the velocity half of each stage
+// derivative is that stage's velocity, itself v0 plus h times a combination
+// of the earlier forces.  Every combination of stage derivatives the step
+// forms is therefore computed from the forces alone, with the combined
+// coefficients of dop853_rkn.hpp (exact products of the tableau, rounded
+// once): the same method and the same values up to rounding, with half the
+// stage arithmetic and half the stage storage of the first-order form.
 template <class F, class Sink>
 DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F& f, Sink& sink) {
     using namespace hw853;
     const double h = J.pos_neg * tmax(std::fabs(L.h), 1e3 * uround);
+    const double hh = h * h;
     const double* y0 = L.y;
-    const double* k1 = L.k1;
-    double yy1[DIM], k2[DIM], k3[DIM], k4[DIM], k5[DIM], k6[DIM], k7[DIM], k8[DIM], k9[DIM],
-        k10[DIM];
+    double Fs[17][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) Fs[1][c] = L.F1[c];
 
-#pragma unroll
-    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * a21 * k1[i];
-    f(yy1, k2);
-#pragma unroll
-    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * (a31 * k1[i] + a32 * k2[i]);
-    f(yy1, k3);
-#pragma unroll
-    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * (a41 * k1[i] + a43 * k3[i]);
-    f(yy1, k4);
-#pragma unroll
-    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * (a51 * k1[i] + a53 * k3[i] + a54 * k4[i]);
-    f(yy1, k5);
-#pragma unroll
-    for (int i = 0; i < DIM; i++) yy1[i] = y0[i] + h * (a61 * k1[i] + a64 * k4[i] + a65 * k5[i]);
-    f(yy1, k6);
-#pragma unroll
-    for (int i = 0; i < DIM; i++)
-        yy1[i] = y0[i] + h * (a71 * k1[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]);
-    f(yy1, k7);
-#pragma unroll
-    for (int i = 0; i < DIM; i++)
-        yy1[i] = y0[i] + h * (a81 * k1[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] + a87 * k7[i]);
-    f(yy1, k8);
-#pragma unroll
-    for (int i = 0; i < DIM; i++)
-        yy1[i] = y0[i] + h * (a91 * k1[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] + a97 * k7[i] +
-                              a98 * k8[i]);
-    f(yy1, k9);
-#pragma unroll
-    for (int i = 0; i < DIM; i++)
-        yy1[i] = y0[i] + h * (a101 * k1[i] + a104 * k4[i] + a105 * k5[i] + a106 * k6[i] +
-                              a107 * k7[i] + a108 * k8[i] + a109 * k9[i]);
-    f(yy1, k10);
-#pragma unroll
-    for (int i = 0; i < DIM; i++)
-        yy1[i] = y0[i] + h * (a111 * k1[i] + a114 * k4[i] + a115 * k5[i] + a116 * k6[i] +
-                              a117 * k7[i] + a118 * k8[i] + a119 * k9[i] + a1110 * k10[i]);
-    f(yy1, k2);
-#pragma unroll
-    for (int i = 0; i < DIM; i++)
-        yy1[i] = y0[i] + h * (a121 * k1[i] + a124 * k4[i] + a125 * k5[i] + a126 * k6[i] +
-                              a127 * k7[i] + a128 * k8[i] + a129 * k9[i] + a1210 * k10[i] +
-                              a1211 * k2[i]);
+    rkn853_stage<2>(J, y0, h, hh, Fs, f);
+    rkn853_stage<3>(J, y0, h, hh, Fs, f);
+    rkn853_stage<4>(J, y0, h, hh, Fs, f);
+    rkn853_stage<5>(J, y0, h, hh, Fs, f);
+    rkn853_stage<6>(J, y0, h, hh, Fs, f);
+    rkn853_stage<7>(J, y0, h, hh, Fs, f);
+    rkn853_stage<8>(J, y0, h, hh, Fs, f);
+    rkn853_stage<9>(J, y0, h, hh, Fs, f);
+    rkn853_stage<10>(J, y0, h, hh, Fs, f);
+    rkn853_stage<11>(J, y0, h, hh, Fs, f);
     const double t_old_older = L.t_old;
     const double t_step = L.t_cur;  // start of this step
     const double t_new = L.t_cur + h;
-    f(yy1, k3);
+    rkn853_stage<12>(J, y0, h, hh, Fs, f);
 
+    // k4 = sum_j b_j k_j and the 8th-order solution k5 = y0 + h k4
+    // (dop853.rs:585-595); the position half of k4 is v0 + h bpos(F)
+    double k4[DIM], k5[DIM];
+    {
+        const Dop853Coef C = dop853_coef(J.coef, Fs[12][2]);
 #pragma unroll
-    for (int i = 0; i < DIM; i++) {
-        k4[i] = b1 * k1[i] + b6 * k6[i] + b7 * k7[i] + b8 * k8[i] + b9 * k9[i] + b10 * k10[i] +
-                b11 * k2[i] + b12 * k3[i];
-        k5[i] = y0[i] + h * k4[i];
+        for (int c = 0; c < 3; c++) {
+            k4[3 + c] = rkn853::bvel(Fs, c, C);
+            k4[c] = y0[3 + c] + h * rkn853::bpos(Fs, c, C);
+        }
     }
+#pragma unroll
+    for (int i = 0; i < DIM; i++) k5[i] = y0[i] + h * k4[i];
+
+    // error estimates (dop853.rs:597-619)
     double err = 0.0, err2 = 0.0;
+    double e1[DIM], e2[DIM];
+    {
+        const Dop853Coef C = dop853_coef(J.coef, k5[5]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            e2[c] = h * rkn853::bhhpos(Fs, c, C);
+            e1[c] = h * rkn853::errpos(Fs, c, C);
+        }
+    }
+    {
+        const Dop853Coef C = dop853_coef(J.coef, e1[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            e2[3 + c] = k4[3 + c] - rkn853::bhhvel(Fs, c, C);
+            e1[3 + c] = rkn853::errvel(Fs, c, C);
+        }
+    }
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
         const double inv_sk = rcp_fast(J.atol + J.rtol * tmax(std::fabs(y0[i]), std::fabs(k5[i])));
-        double erri = k4[i] - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i];
-        double sqr = erri * inv_sk;
+        double sqr = e2[i] * inv_sk;
         err2 += sqr * sqr;
-        erri = er1 * k1[i] + er6 * k6[i] + er7 * k7[i] + er8 * k8[i] + er9 * k9[i] +
-               er10 * k10[i] + er11 * k2[i] + er12 * k3[i];
-        sqr = erri * inv_sk;
+        sqr = e1[i] * inv_sk;
         err += sqr * sqr;
     }
     double deno = err + 0.01 * err2;
@@ -327,57 +390,91 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
 
     if (err <= 1.0) {
         L.accepted++;
-        f(k5, k4);  // FSAL derivative at the new point
+        // FSAL derivative at the new point (stage 13): (v_new, F(x_new))
+        {
+            double a[DIM];
+            f(k5, a);
+#pragma unroll
+            for (int c = 0; c < 3; c++) Fs[13][c] = a[3 + c];
+        }
         const double* times = J.times;
         const int32_t last = J.nt - 1;
         if (L.ii < last && J.pos_neg * L.t_next < J.pos_neg * t_new) {
             // Dense output (dop853.rs:636-763), only when a sample falls in
-            // (t_step, t_new).
+            // (t_step, t_new).  rcont5..8 are kept without their factor h
+            // (applied per sample); their position halves are
+            // h dense_k(F), the velocity halves sum_j d_kj F_j.
             L.dense++;
             double r1[DIM], r2[DIM], r3[DIM], r4[DIM], r5[DIM], r6[DIM], r7[DIM], r8[DIM];
 #pragma unroll
             for (int i = 0; i < DIM; i++) {
+                // k1 = (v0, F1), k4 (the new derivative) = (v_new, F13)
+                const double k1i = i < 3 ? y0[3 + i] : Fs[1][i - 3];
+                const double k4i = i < 3 ? k5[3 + i] : Fs[13][i - 3];
                 r1[i] = y0[i];
                 double ydiff = k5[i] - y0[i];
                 r2[i] = ydiff;
-                double bspl = h * k1[i] - ydiff;
+                double bspl = h * k1i - ydiff;
                 r3[i] = bspl;
-                r4[i] = ydiff - h * k4[i] - bspl;
-                r5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
-                        d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
-                r6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
-                        d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
-                r7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
-                        d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
-                r8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
-                        d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
+                r4[i] = ydiff - h * k4i - bspl;
             }
+            // one coefficient group (16 values) at a time: the groups'
+            // scalar loads then fit the scalar registers
+            {
+                const Dop853Coef C = dop853_coef(J.coef, Fs[13][2]);
 #pragma unroll
-            for (int i = 0; i < DIM; i++)
-                yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
-                                      a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] +
-                                      a1413 * k4[i]);
-            f(yy1, k10);
+                for (int c = 0; c < 3; c++) {
+                    r5[c] = rkn853::dense4pos_pre(Fs, c, C);
+                    r5[3 + c] = rkn853::dense4vel_pre(Fs, c, C);
+                }
+            }
+            {
+                const Dop853Coef C = dop853_coef(J.coef, r5);
 #pragma unroll
-            for (int i = 0; i < DIM; i++)
-                yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
-                                      a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] +
-                                      a1514 * k10[i]);
-            f(yy1, k2);
+                for (int c = 0; c < 3; c++) {
+                    r6[c] = rkn853::dense5pos_pre(Fs, c, C);
+                    r6[3 + c] = rkn853::dense5vel_pre(Fs, c, C);
+                }
+            }
+            {
+                const Dop853Coef C = dop853_coef(J.coef, r6);
 #pragma unroll
-            for (int i = 0; i < DIM; i++)
-                yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
-                                      a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] +
-                                      a1615 * k2[i]);
-            f(yy1, k3);
-            // rcont5..8 without their common factor h (dop853.rs:743-746);
-            // h is applied once per sample, as s1 * h, below
+                for (int c = 0; c < 3; c++) {
+                    r7[c] = rkn853::dense6pos_pre(Fs, c, C);
+                    r7[3 + c] = rkn853::dense6vel_pre(Fs, c, C);
+                }
+            }
+            {
+                const Dop853Coef C = dop853_coef(J.coef, r7);
 #pragma unroll
-            for (int i = 0; i < DIM; i++) {
-                r5[i] = r5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i];
-                r6[i] = r6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i];
-                r7[i] = r7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i];
-                r8[i] = r8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i];
+                for (int c = 0; c < 3; c++) {
+                    r8[c] = rkn853::dense7pos_pre(Fs, c, C);
+                    r8[3 + c] = rkn853::dense7vel_pre(Fs, c, C);
+                }
+            }
+            // the three extra stages (dop853.rs:703-742)
+            rkn853_stage<14>(J, y0, h, hh, Fs, f);
+            rkn853_stage<15>(J, y0, h, hh, Fs, f);
+            rkn853_stage<16>(J, y0, h, hh, Fs, f);
+            {
+                const Dop853Coef C = dop853_coef(J.coef, Fs[16][2]);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    r5[c] += rkn853::dense4pos_post(Fs, c, C);
+                    r6[c] += rkn853::dense5pos_post(Fs, c, C);
+                    r5[3 + c] += rkn853::dense4vel_post(Fs, c, C);
+                    r6[3 + c] += rkn853::dense5vel_post(Fs, c, C);
+                }
+            }
+            {
+                const Dop853Coef C = dop853_coef(J.coef, r6);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    r7[c] += rkn853::dense6pos_post(Fs, c, C);
+                    r8[c] += rkn853::dense7pos_post(Fs, c, C);
+                    r7[3 + c] += rkn853::dense6vel_post(Fs, c, C);
+                    r8[3 + c] += rkn853::dense7vel_post(Fs, c, C);
+                }
             }
             const double inv_h = rcp_fast(h);
             double t_s = L.t_next;
@@ -389,13 +486,16 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
                 const double sv = (t_s - t_step) * inv_h;
                 const double s1 = 1.0 - sv;
                 const double s1h = s1 * h;
+                const double s1hh = s1h * h;  // positions carry one more h
                 double yy[DIM];
 #pragma unroll
-                for (int i = 0; i < DIM; i++)
+                for (int i = 0; i < DIM; i++) {
+                    const double w = i < 3 ? s1hh : s1h;
                     yy[i] = r1[i] +
                             sv * (r2[i] +
                                   s1 * (r3[i] +
-                                        sv * (r4[i] + s1h * (r5[i] + sv * (r6[i] + s1 * (r7[i] + sv * r8[i]))))));
+                                        sv * (r4[i] + w * (r5[i] + sv * (r6[i] + s1 * (r7[i] + sv * r8[i]))))));
+                }
                 L.ii++;
                 if (!finite6(yy)) L.status |= STATUS_NONFINITE;
                 sink.emit(L.ii, yy);
@@ -404,10 +504,9 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
             L.t_next = t_s;
         }
 #pragma unroll
-        for (int i = 0; i < DIM; i++) {
-            L.k1[i] = k4[i];
-            L.y[i] = k5[i];
-        }
+        for (int i = 0; i < DIM; i++) L.y[i] = k5[i];
+#pragma unroll
+        for (int c = 0; c < 3; c++) L.F1[c] = Fs[13][c];
         hnew = std::fabs(hnew) > std::fabs(J.hmax) ? J.pos_neg * J.hmax : hnew;
         if (L.reject != 0) hnew = J.pos_neg * tmin(std::fabs(hnew), std::fabs(h));
         L.reject = 0;

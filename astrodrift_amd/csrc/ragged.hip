@@ -100,7 +100,7 @@ __device__ __forceinline__ void save(const Resident& r, const Dop853Lane& L, int
 #pragma unroll
     for (int k = 0; k < DIM; k++) {
         r.w(k) = L.y[k];
-        r.w(6 + k) = L.k1[k];
+        if (k < 3) r.w(6 + k) = L.F1[k];
     }
     r.w(12) = L.h;
     r.w(13) = L.t_cur;
@@ -115,7 +115,7 @@ __device__ __forceinline__ void restore(const Resident& r, Dop853Lane& L) {
 #pragma unroll
     for (int k = 0; k < DIM; k++) {
         L.y[k] = r.w(k);
-        L.k1[k] = r.w(6 + k);
+        if (k < 3) L.F1[k] = r.w(6 + k);
     }
     L.h = r.w(12);
     L.t_cur = r.w(13);

@@ -1,0 +1,168 @@
+"""The DOP853 step machine's second-order (Runge-Kutta-Nystrom) restatement.
+
+* dop853_rkn.hpp is what gen_dop853_rkn.py writes from the tableau.
+* Every generated combination equals the first-order form it replaces: with
+  random forces F_m and a random v0, the stage velocities V_j = v0 + h sum_m
+  a_jm F_m are built directly from the tableau, and sum_j w_j V_j (stage
+  positions, update, error estimates, dense coefficients) is compared with the
+  generated sum over the forces.  A wrong index or coefficient in the
+  generated table fails this.
+* The product's step machine, compiled for the host with g++, reproduces the
+  oracle on the 1M-orbit Milky-Way workload's strided subset and on Kepler
+  and Plummer orbits (the same 1e-8 bar as the GPU parity tests), with the
+  same accepted-step counts to 0.1%.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+import subprocess
+import sys
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from astrodrift_amd import _native, workloads
+
+REPO = Path(__file__).resolve().parent.parent
+CSRC = REPO / "astrodrift_amd" / "csrc"
+
+
+def test_header_is_generated_from_tableau():
+    r = subprocess.run([sys.executable, str(CSRC / "gen_dop853_rkn.py"), "--check"])
+    assert r.returncode == 0, "dop853_rkn.hpp differs from gen_dop853_rkn.py's output"
+
+
+def _tableau():
+    src = (CSRC / "integrators.hpp").read_text()
+    ns = src[src.index("namespace hw853 {"):src.index("}  // namespace hw853")]
+    return {k: float(Fraction(v)) for k, v in re.findall(r"\b([a-z]+\d+)\s*=\s*([-0-9.e]+)", ns)}
+
+
+def _header():
+    """kCoef values and, per generated function, its [(index, stage)] terms."""
+    text = (CSRC / "dop853_rkn.hpp").read_text()
+    block = text[text.index("constexpr double kCoef[kNumCoef] = {"):text.index("};")]
+    coef = [float(m) for m in re.findall(r"^\s+([-0-9.e]+),\s+//", block, re.M)]
+    funcs = {}
+    for name, body in re.findall(r"double (\w+)\(const FA& F, int c, CP C\) \{\s+return ([^;]+);",
+                                 text):
+        funcs[name] = [(int(i), int(m)) for i, m in re.findall(r"C\[(\d+)\] \* F\[(\d+)\]\[c\]",
+                                                               body)]
+    kc = {int(i): int(v) for i, v in re.findall(r"constexpr int k_c(\d+) = (\d+);", text)}
+    return coef, funcs, kc
+
+
+def test_generated_sums_equal_first_order_form():
+    t = _tableau()
+    coef, funcs, kc = _header()
+    b = {j: t.get(f"b{j}", 0.0) for j in range(1, 13)}
+
+    def A(i, j):
+        return b.get(j, 0.0) if i == 13 else t.get(f"a{i}{j}", 0.0)
+
+    rng = np.random.default_rng(5)
+    h = 0.37
+    for trial in range(5):
+        F = {m: rng.normal(size=3) for m in range(1, 17)}
+        v0 = rng.normal(size=3)
+        x0 = rng.normal(size=3)
+        V = {1: v0}
+        for j in range(2, 17):
+            V[j] = v0 + h * sum(A(j, m) * F[m] for m in range(1, j))
+
+        def gen(name):
+            return sum(coef[i] * F[m] for i, m in funcs.get(name, [])) + np.zeros(3)
+
+        def close(a, ref):
+            scale = max(1.0, float(np.max(np.abs(ref))))
+            assert np.max(np.abs(a - ref)) < 1e-12 * scale, (a, ref)
+
+        for i in list(range(2, 13)) + [14, 15, 16]:
+            direct = x0 + h * sum(A(i, j) * V[j] for j in range(1, i))
+            rkn = x0 + h * coef[kc[i]] * v0 + h * h * gen(f"stage{i}")
+            close(rkn, direct)
+        # update: sum_j b_j k_j = (sum b V, sum b F)
+        close(v0 + h * gen("bpos"), sum(b[j] * V[j] for j in b))
+        close(gen("bvel"), sum(b[j] * F[j] for j in b))
+        er = {j: t.get(f"er{j}", 0.0) for j in range(1, 13)}
+        close(h * gen("errpos"), sum(er[j] * V[j] for j in er))
+        close(gen("errvel"), sum(er[j] * F[j] for j in er))
+        k4x = sum(b[j] * V[j] for j in b)
+        close(h * gen("bhhpos"), k4x - t["bhh1"] * V[1] - t["bhh2"] * V[9] - t["bhh3"] * V[12])
+        close(gen("bhhvel"), t["bhh1"] * F[1] + t["bhh2"] * F[9] + t["bhh3"] * F[12])
+        for k in (4, 5, 6, 7):
+            d = {j: t.get(f"d{k}{j}", 0.0) for j in range(1, 17)}
+            pos = h * (gen(f"dense{k}pos_pre") + gen(f"dense{k}pos_post"))
+            vel = gen(f"dense{k}vel_pre") + gen(f"dense{k}vel_post")
+            close(pos, sum(d[j] * V[j] for j in d if d[j] != 0.0))
+            close(vel, sum(d[j] * F[j] for j in d if d[j] != 0.0))
+
+
+@pytest.fixture(scope="module")
+def host_lib(tmp_path_factory):
+    out = tmp_path_factory.mktemp("host") / "host_step_machine.so"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
+           "-I/opt/rocm/include", f"-I{CSRC}", f"-I{REPO / 'include'}",
+           str(REPO / "tests" / "host_step_machine.cpp"), str(CSRC / "spherical_table.cpp"),
+           "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip(f"host build of the step machine failed: {r.stderr[-400:]}")
+    lib = ctypes.CDLL(str(out))
+    lib.host_dop853.restype = ctypes.c_int
+    lib.host_dop853.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                ctypes.c_void_p]
+    return lib
+
+
+def _host(lib, spec, y0, times, rtol, atol):
+    y0 = np.ascontiguousarray(y0, dtype=np.float64)
+    n, nt = y0.shape[0], times.shape[0]
+    out = np.zeros((nt, n, 6))
+    st = np.zeros((n, 4), np.int64)
+    pot = spec.to_c() if spec is not None else None
+    assert lib.host_dop853(ctypes.byref(pot) if pot is not None else None, n, y0.ctypes.data,
+                           times.ctypes.data, nt, rtol, atol, out.ctypes.data, st.ctypes.data) == 0
+    return out, st
+
+
+def _norm_err(a, ref):
+    dp = np.linalg.norm(a[..., :3] - ref[..., :3], axis=-1) / np.linalg.norm(ref[..., :3], axis=-1)
+    dv = np.linalg.norm(a[..., 3:] - ref[..., 3:], axis=-1) / np.linalg.norm(ref[..., 3:], axis=-1)
+    return float(dp.max()), float(dv.max())
+
+
+def test_host_step_machine_matches_oracle_mw(host_lib):
+    w = workloads.mw_disk()
+    idx = np.arange(0, w.n, w.n // 64)
+    y0 = w.state0[idx]
+    out, st = _host(host_lib, w.potential, y0, w.times, w.rtol, w.atol)
+    tol = math.log(w.rtol)
+    ref, rst = oracle_lib.batch(1, y0, w.times, tol, tol, w.potential, nthreads=8)
+    ep, ev = _norm_err(out, ref)
+    print(f"MW subset: pos {ep:.2e} vel {ev:.2e}, accepted {st[:, 0].sum()} vs "
+          f"{rst['accepted'].sum()}")
+    assert ep < 1e-8 and ev < 1e-8
+    assert abs(int(st[:, 0].sum()) - int(rst["accepted"].sum())) <= 1e-3 * rst["accepted"].sum()
+    assert np.array_equal(out[0], y0)
+
+
+@pytest.mark.parametrize("pot", ["galpy", "kepler", "plummer"])
+def test_host_step_machine_matches_oracle_point_masses(host_lib, pot):
+    spec = {"galpy": None, "kepler": _native.PotentialSpec.kepler(1.3),
+            "plummer": _native.PotentialSpec.plummer(1.0, 0.1)}[pot]
+    k = workloads.kepler_disk(n=64)
+    out, st = _host(host_lib, spec, k.state0, k.times, k.rtol, k.atol)
+    tol = math.log(k.rtol)
+    ref, rst = oracle_lib.batch(1, k.state0, k.times, tol, tol, spec)
+    ep, ev = _norm_err(out, ref)
+    print(f"{pot}: pos {ep:.2e} vel {ev:.2e}")
+    assert ep < 1e-8 and ev < 1e-8
+    assert abs(int(st[:, 0].sum()) - int(rst["accepted"].sum())) <= 1e-3 * rst["accepted"].sum()
