@@ -634,6 +634,61 @@ template <class FA, class CP>
 DRIFT_HD DRIFT_FI double dense7pos_post(const FA& F, int c, CP C) {
     return C[222] * F[13][c] + C[223] * F[14][c] + C[224] * F[15][c];
 }
+template <class FA>
+DRIFT_HD DRIFT_FI double stage3_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage4_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[2][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage5_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[2][c] + L[3] * F[3][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage6_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[3][c] + L[3] * F[4][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage7_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[3][c] + L[3] * F[4][c] + L[4] * F[5][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage8_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[3][c] + L[3] * F[4][c] + L[4] * F[5][c] + L[5] * F[6][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage9_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[3][c] + L[3] * F[4][c] + L[4] * F[5][c] + L[5] * F[6][c] + L[6] * F[7][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage10_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[3][c] + L[3] * F[4][c] + L[4] * F[5][c] + L[5] * F[6][c] + L[6] * F[7][c] + L[7] * F[8][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage11_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[3][c] + L[3] * F[4][c] + L[4] * F[5][c] + L[5] * F[6][c] + L[6] * F[7][c] + L[7] * F[8][c] + L[8] * F[9][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage12_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[3][c] + L[3] * F[4][c] + L[4] * F[5][c] + L[5] * F[6][c] + L[6] * F[7][c] + L[7] * F[8][c] + L[8] * F[9][c] + L[9] * F[10][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage14_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[6][c] + L[3] * F[7][c] + L[4] * F[8][c] + L[5] * F[9][c] + L[6] * F[10][c] + L[7] * F[11][c] + L[8] * F[12][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage15_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[6][c] + L[3] * F[7][c] + L[4] * F[8][c] + L[5] * F[9][c] + L[6] * F[10][c] + L[7] * F[11][c] + L[8] * F[12][c] + L[9] * F[13][c];
+}
+template <class FA>
+DRIFT_HD DRIFT_FI double stage16_local(const FA& F, int c, const double* L) {
+    return L[1] * F[1][c] + L[2] * F[6][c] + L[3] * F[7][c] + L[4] * F[8][c] + L[5] * F[9][c] + L[6] * F[10][c] + L[7] * F[11][c] + L[8] * F[12][c] + L[9] * F[13][c] + L[10] * F[14][c];
+}
+
+// number of coefficients of stage i (c_i and its terms), from kStageC[i]
+constexpr int kStageN[17] = {0, 0, 1, 2, 3, 4, 4, 5, 6, 7, 8, 9, 10, 0, 9, 10, 11};
 
 // index of c_i in kCoef for stage i (-1: no such stage)
 constexpr int kStageC[17] = {-1, -1, k_c2, k_c3, k_c4, k_c5, k_c6, k_c7, k_c8, k_c9, k_c10, k_c11, k_c12, -1, k_c14, k_c15, k_c16};
@@ -655,6 +710,25 @@ DRIFT_HD DRIFT_FI double stage_sum(const FA& F, int c, CP C) {
     if constexpr (I == 14) return stage14(F, c, C);
     if constexpr (I == 15) return stage15(F, c, C);
     if constexpr (I == 16) return stage16(F, c, C);
+}
+
+// the same from a local copy L of the stage's coefficients (L[0] = c_I)
+template <int I, class FA>
+DRIFT_HD DRIFT_FI double stage_sum_local(const FA& F, int c, const double* L) {
+    if constexpr (I == 2) return 0.0;
+    if constexpr (I == 3) return stage3_local(F, c, L);
+    if constexpr (I == 4) return stage4_local(F, c, L);
+    if constexpr (I == 5) return stage5_local(F, c, L);
+    if constexpr (I == 6) return stage6_local(F, c, L);
+    if constexpr (I == 7) return stage7_local(F, c, L);
+    if constexpr (I == 8) return stage8_local(F, c, L);
+    if constexpr (I == 9) return stage9_local(F, c, L);
+    if constexpr (I == 10) return stage10_local(F, c, L);
+    if constexpr (I == 11) return stage11_local(F, c, L);
+    if constexpr (I == 12) return stage12_local(F, c, L);
+    if constexpr (I == 14) return stage14_local(F, c, L);
+    if constexpr (I == 15) return stage15_local(F, c, L);
+    if constexpr (I == 16) return stage16_local(F, c, L);
 }
 
 }  // namespace rkn853

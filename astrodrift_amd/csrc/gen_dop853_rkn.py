@@ -142,6 +142,22 @@ def emit(sums) -> str:
     for name, terms, c in sums:
         if name.startswith("stage"):
             kc[int(name[5:])] = f"k_c{int(name[5:])}"
+    # stage sums over a local copy of the stage's coefficients (c_i, then the
+    # terms in order), for kernels that fetch a stage's coefficients ahead
+    kn = ["0"] * 17
+    for name, terms, c in sums:
+        if name.startswith("stage"):
+            i = int(name[5:])
+            kn[i] = str(1 + len(terms))
+            if terms:
+                expr = " + ".join(f"L[{1 + q}] * F[{m}][c]" for q, (m, _) in enumerate(terms))
+                funcs.append("template <class FA>")
+                funcs.append(f"DRIFT_HD DRIFT_FI double {name}_local(const FA& F, int c, const double* L) {{")
+                funcs.append(f"    return {expr};")
+                funcs.append("}")
+    funcs.append("")
+    funcs.append("// number of coefficients of stage i (c_i and its terms), from kStageC[i]")
+    funcs.append("constexpr int kStageN[17] = {" + ", ".join(kn) + "};")
     funcs.append("")
     funcs.append("// index of c_i in kCoef for stage i (-1: no such stage)")
     funcs.append("constexpr int kStageC[17] = {" + ", ".join(kc) + "};")
@@ -152,6 +168,15 @@ def emit(sums) -> str:
     for i in stage_ids:
         has = any(name == f"stage{i}" and terms for name, terms, _ in sums)
         body = f"return stage{i}(F, c, C);" if has else "return 0.0;"
+        funcs.append(f"    if constexpr (I == {i}) {body}")
+    funcs.append("}")
+    funcs.append("")
+    funcs.append("// the same from a local copy L of the stage's coefficients (L[0] = c_I)")
+    funcs.append("template <int I, class FA>")
+    funcs.append("DRIFT_HD DRIFT_FI double stage_sum_local(const FA& F, int c, const double* L) {")
+    for i in stage_ids:
+        has = any(name == f"stage{i}" and terms for name, terms, _ in sums)
+        body = f"return stage{i}_local(F, c, L);" if has else "return 0.0;"
         funcs.append(f"    if constexpr (I == {i}) {body}")
     funcs.append("}")
     lines = [

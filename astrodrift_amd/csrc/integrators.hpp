@@ -275,23 +275,33 @@ DRIFT_HD DRIFT_FI void abandon_orbit(int32_t& ii, int32_t nt, int32_t& status, i
     status |= flag;
 }
 
+// Copies stage I's coefficients (c_I and its (A^2)_Im terms) out of the table.
+template <int I>
+DRIFT_HD DRIFT_FI void rkn853_load(Dop853Coef C, double* L) {
+#pragma unroll
+    for (int q = 0; q < rkn853::kStageN[I]; q++) L[q] = C[rkn853::kStageC[I] + q];
+}
+
 // Evaluates stage I of the Runge-Kutta-Nystrom form: the stage position
 // X_I = x0 + h c_I v0 + h^2 sum_m (A^2)_Im F_m (the stage velocity, which the
-// force does not read, is never formed) and its force F_I.
-template <int I, class F>
+// force does not read, is never formed) and its force F_I.  L holds stage I's
+// coefficients; stage NEXT's are fetched into Lnext before the force call,
+// so their scalar loads overlap the force evaluation.
+template <int I, int NEXT, class F>
 DRIFT_HD DRIFT_FI void rkn853_stage(const Dop853Job& J, const double* y0, double h, double hh,
-                                    double (&Fs)[17][3], const F& f) {
-    const Dop853Coef C = dop853_coef(J.coef, Fs[I - 1][2]);
-    const double hc = h * C[rkn853::kStageC[I]];
+                                    double (&Fs)[17][3], const double* L, double* Lnext,
+                                    const F& f) {
+    const double hc = h * L[0];
     double q[DIM], a[DIM];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         if constexpr (I == 2)
             q[c] = y0[c] + hc * y0[3 + c];
         else
-            q[c] = y0[c] + (hc * y0[3 + c] + hh * rkn853::stage_sum<I>(Fs, c, C));
+            q[c] = y0[c] + (hc * y0[3 + c] + hh * rkn853::stage_sum_local<I>(Fs, c, L));
         q[3 + c] = 0.0;
     }
+    if constexpr (NEXT > 0) rkn853_load<NEXT>(dop853_coef(J.coef, q[2]), Lnext);
     f(q, a);
 #pragma unroll
     for (int c = 0; c < 3; c++) Fs[I][c] = a[3 + c];
@@ -317,20 +327,23 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
 #pragma unroll
     for (int c = 0; c < 3; c++) Fs[1][c] = L.F1[c];
 
-    rkn853_stage<2>(J, y0, h, hh, Fs, f);
-    rkn853_stage<3>(J, y0, h, hh, Fs, f);
-    rkn853_stage<4>(J, y0, h, hh, Fs, f);
-    rkn853_stage<5>(J, y0, h, hh, Fs, f);
-    rkn853_stage<6>(J, y0, h, hh, Fs, f);
-    rkn853_stage<7>(J, y0, h, hh, Fs, f);
-    rkn853_stage<8>(J, y0, h, hh, Fs, f);
-    rkn853_stage<9>(J, y0, h, hh, Fs, f);
-    rkn853_stage<10>(J, y0, h, hh, Fs, f);
-    rkn853_stage<11>(J, y0, h, hh, Fs, f);
+    // stage coefficients, double-buffered: each stage fetches the next one's
+    double Lc0[12], Lc1[12];
+    rkn853_load<2>(dop853_coef(J.coef, h), Lc0);
+    rkn853_stage<2, 3>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<3, 4>(J, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<4, 5>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<5, 6>(J, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<6, 7>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<7, 8>(J, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<8, 9>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<9, 10>(J, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<10, 11>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<11, 12>(J, y0, h, hh, Fs, Lc1, Lc0, f);
     const double t_old_older = L.t_old;
     const double t_step = L.t_cur;  // start of this step
     const double t_new = L.t_cur + h;
-    rkn853_stage<12>(J, y0, h, hh, Fs, f);
+    rkn853_stage<12, 0>(J, y0, h, hh, Fs, Lc0, Lc1, f);
 
     // k4 = sum_j b_j k_j and the 8th-order solution k5 = y0 + h k4
     // (dop853.rs:585-595); the position half of k4 is v0 + h bpos(F)
@@ -390,7 +403,9 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
 
     if (err <= 1.0) {
         L.accepted++;
-        // FSAL derivative at the new point (stage 13): (v_new, F(x_new))
+        // FSAL derivative at the new point (stage 13): (v_new, F(x_new)); the
+        // first dense stage's coefficients are fetched meanwhile
+        rkn853_load<14>(dop853_coef(J.coef, k5[2]), Lc0);
         {
             double a[DIM];
             f(k5, a);
@@ -453,9 +468,9 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
                 }
             }
             // the three extra stages (dop853.rs:703-742)
-            rkn853_stage<14>(J, y0, h, hh, Fs, f);
-            rkn853_stage<15>(J, y0, h, hh, Fs, f);
-            rkn853_stage<16>(J, y0, h, hh, Fs, f);
+            rkn853_stage<14, 15>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+            rkn853_stage<15, 16>(J, y0, h, hh, Fs, Lc1, Lc0, f);
+            rkn853_stage<16, 0>(J, y0, h, hh, Fs, Lc0, Lc1, f);
             {
                 const Dop853Coef C = dop853_coef(J.coef, Fs[16][2]);
 #pragma unroll
