@@ -957,7 +957,16 @@ int drift_tune_device(drift_session* s, int32_t method, const double* d_state0,
     drift_launch_options chosen = base;
     float chosen_ms = 0.0f;
     bool have = false;
+    // A trial covers only a prefix of the time grid, where the plain
+    // schedule's tail (each wavefront waits for its slowest orbit, the grid
+    // for its slowest block) is short; over the whole grid the persistent
+    // schedule wins whenever its lanes integrate several orbits each.  So
+    // once n reaches twice the resident lanes (2 waves of 64 per SIMD, 4 SIMDs
+    // per CU) only persistent launches are tried.
+    const size_t resident_lanes = (size_t)std::max(s->cu_count, 1) * 4 * 2 * 64;
+    const bool try_plain = n < 2 * resident_lanes;
     for (int schedule : {DRIFT_SCHEDULE_PLAIN, DRIFT_SCHEDULE_PERSISTENT}) {
+        if (schedule == DRIFT_SCHEDULE_PLAIN && !try_plain) continue;
         for (int block : {64, 128, 256}) {
             for (int ordering : {DRIFT_ORDER_INDEX, DRIFT_ORDER_COST}) {
                 drift_launch_options o = base;

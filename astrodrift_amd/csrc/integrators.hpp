@@ -296,9 +296,9 @@ DRIFT_HD DRIFT_FI void rkn853_stage(const Dop853Job& J, const double* y0, double
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         if constexpr (I == 2)
-            q[c] = y0[c] + hc * y0[3 + c];
+            q[c] = fma(hc, y0[3 + c], y0[c]);
         else
-            q[c] = y0[c] + (hc * y0[3 + c] + hh * rkn853::stage_sum_local<I>(Fs, c, L));
+            q[c] = fma(hh, rkn853::stage_sum_local<I>(Fs, c, L), fma(hc, y0[3 + c], y0[c]));
         q[3 + c] = 0.0;
     }
     if constexpr (NEXT > 0) rkn853_load<NEXT>(dop853_coef(J.coef, q[2]), Lnext);

@@ -191,8 +191,10 @@ int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
 
 /* Autotuner (no counterpart in the reference).  Times every candidate
  * {PLAIN, PERSISTENT} x block {64, 128, 256} x ordering {INDEX, COST} on the
- * launch described by the arguments, integrated over the first prefix_samples
- * output times only (0 = max(2, nt / 20)) so that every orbit takes part, and
+ * launch described by the arguments (PLAIN only while n is below twice the
+ * device's resident lanes: a prefix hides the plain schedule's tail), each
+ * integrated over the first prefix_samples output times only
+ * (0 = max(2, nt / 20)) so that every orbit takes part, and
  * returns the fastest in *best (precision and max_attempts as given in
  * *best on entry, or 0) and its time in *best_ms (nullable).  The choice is
  * stored in the session under (method, potential kind, precision, log2 n) for
