@@ -43,8 +43,11 @@ SCHEDULE_TUNED = 2  # what tune_device stored for this method / potential / prec
 ORDER_AUTO = 0   # cost order for persistent launches of >= 16384 orbits
 ORDER_INDEX = 1
 ORDER_COST = 2
+STAGING_AUTO = 0
+STAGING_DIRECT = 1
+STAGING_STAGED = 2
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 STATUS_CODE_MASK = 0xFF
 STATUS_NONFINITE = 0x100
@@ -104,7 +107,7 @@ class CLaunchOptions(ctypes.Structure):
         ("schedule", ctypes.c_int32),
         ("max_attempts", ctypes.c_uint32),
         ("ordering", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("staging", ctypes.c_int32),
     ]
 
 
@@ -426,10 +429,11 @@ def launch_device(
     dt_one: float = INITIAL_STEP_SENTINEL,
     device: int = 0,
     ordering: int = ORDER_AUTO,
+    staging: int = STAGING_AUTO,
 ) -> None:
     """Device-resident launch on raw device pointers (e.g. torch data_ptr())."""
     s = session(device)
-    opts = CLaunchOptions(precision, block_size, schedule, max_attempts, ordering, 0)
+    opts = CLaunchOptions(precision, block_size, schedule, max_attempts, ordering, staging)
     pot = potential.to_c() if potential is not None else None
     pot_ref = ctypes.byref(pot) if pot is not None else None
     args = (s.handle, ctypes.c_void_p(d_state0), ctypes.c_void_p(d_times), ctypes.c_void_p(d_out),
@@ -458,7 +462,7 @@ def tune_device(method: int, d_state0: int, d_times: int, n: int, nt: int, rtol_
                                   ctypes.byref(pot) if pot is not None else None, prefix_samples,
                                   ctypes.byref(best), ctypes.byref(ms), ctypes.c_void_p(stream)))
     return {"schedule": best.schedule, "block_size": best.block_size,
-            "ordering": best.ordering, "ms": ms.value}
+            "ordering": best.ordering, "staging": best.staging, "ms": ms.value}
 
 
 @dataclass

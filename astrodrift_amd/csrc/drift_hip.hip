@@ -30,18 +30,31 @@ namespace drift {
 // Device side
 // ---------------------------------------------------------------------------
 
+// Output staging (DRIFT_STAGING_STAGED): samples in blocks of kStageBlock
+// consecutive samples per orbit, block b of orbit i at
+// ((b * n) + i) * kStageBlock records, so a lane writes its orbit's records in
+// order and whole lines; unstage_kernel then writes the time-major output.
+constexpr int kStageShift = 4;
+constexpr int kStageBlock = 1 << kStageShift;
+
+__device__ __forceinline__ int64_t out_record(int64_t s, int64_t n, int64_t idx, bool staged) {
+    if (!staged) return s * n + idx;
+    return (((s >> kStageShift) * n + idx) << kStageShift) + (s & (kStageBlock - 1));
+}
+
 struct OutSink {
     double* out;
     int64_t n;
     int64_t idx;
+    bool staged;
     __device__ __forceinline__ void emit(int32_t s, const double* y) const {
-        double2* p = reinterpret_cast<double2*>(out + ((int64_t)s * n + idx) * DIM);
+        double2* p = reinterpret_cast<double2*>(out + out_record(s, n, idx, staged) * DIM);
         p[0] = make_double2(y[0], y[1]);
         p[1] = make_double2(y[2], y[3]);
         p[2] = make_double2(y[4], y[5]);
     }
     __device__ __forceinline__ void emit_component(int32_t s, int c, double v) const {
-        out[((int64_t)s * n + idx) * DIM + c] = v;
+        out[out_record(s, n, idx, staged) * DIM + c] = v;
     }
 };
 
@@ -60,6 +73,7 @@ struct PlainArgs {
     const void* table;    // MW: TABLE_DOUBLES coefficients (double or float)
     unsigned long long* queue;  // persistent schedule: next unclaimed orbit slot
     const uint32_t* order;      // slot -> orbit (NULL = identity)
+    int32_t staged;             // out is the orbit-blocked staging buffer
 };
 
 template <int KIND, class FT>
@@ -125,7 +139,7 @@ __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_plain_ke
     const double t0 = a.times[0];
     const Dop853Job J = make_dop853_job(a.times, a.nt, a.rtol, a.atol, a.max_attempts, t0,
                                         a.times[a.nt - 1]);
-    OutSink sink{a.out, a.n, i};
+    OutSink sink{a.out, a.n, i, a.staged != 0};
     double y0[DIM];
     load_state(a.state0, i, y0);
     Dop853Lane L;
@@ -160,7 +174,7 @@ __global__ void __launch_bounds__(kDefaultBlock) dopr54_plain_kernel(PlainArgs a
     J.dt_one_init = a.dt_one_init;
 #pragma unroll
     for (int k = 0; k < 6; k++) J.bound2[k] = a.bound2[k];
-    OutSink sink{a.out, a.n, i};
+    OutSink sink{a.out, a.n, i, a.staged != 0};
     double y0[DIM];
     load_state(a.state0, i, y0);
     Dopr54Lane L;
@@ -207,7 +221,7 @@ __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_persiste
     const double t0 = a.times[0];
     const Dop853Job J = make_dop853_job(a.times, a.nt, a.rtol, a.atol, a.max_attempts, t0,
                                         a.times[a.nt - 1]);
-    OutSink sink{a.out, a.n, 0};
+    OutSink sink{a.out, a.n, 0, a.staged != 0};
     Dop853Lane L;
     bool need = true;
     for (;;) {
@@ -249,7 +263,7 @@ __global__ void __launch_bounds__(kDefaultBlock) dopr54_persistent_kernel(PlainA
     J.dt_one_init = a.dt_one_init;
 #pragma unroll
     for (int k = 0; k < 6; k++) J.bound2[k] = a.bound2[k];
-    OutSink sink{a.out, a.n, 0};
+    OutSink sink{a.out, a.n, 0, a.staged != 0};
     Dopr54Lane L;
     bool need = true;
     for (;;) {
@@ -280,6 +294,40 @@ __global__ void __launch_bounds__(kDefaultBlock) dopr54_persistent_kernel(PlainA
 }
 
 __global__ void reset_queue_kernel(unsigned long long* queue) { *queue = 0ull; }
+
+// Staged samples -> time-major output.  One block moves a tile of 64 orbits x
+// kStageBlock samples: it reads the tile's staged records (contiguous, 48 KB
+// for a full tile), and writes each of its samples as 64 consecutive records
+// (3 KB) of the output.  The LDS rows are padded by one 16-byte word to spread
+// the column reads over the banks.
+constexpr int kUnstageOrbits = 64;
+constexpr int kUnstageRow = kStageBlock * 3;  // double2 words per orbit in a tile
+
+__global__ void __launch_bounds__(256) unstage_kernel(const double* __restrict__ staged,
+                                                      double* __restrict__ out, int64_t n,
+                                                      int32_t nt) {
+    __shared__ double2 tile[kUnstageOrbits][kUnstageRow + 1];
+    const int64_t o0 = (int64_t)blockIdx.x * kUnstageOrbits;
+    const int64_t orbits = min((int64_t)kUnstageOrbits, n - o0);
+    const int32_t nblocks = (nt + kStageBlock - 1) / kStageBlock;
+    for (int32_t b = blockIdx.y; b < nblocks; b += gridDim.y) {
+        const double2* src = reinterpret_cast<const double2*>(staged) +
+                             ((int64_t)b * n + o0) * kStageBlock * 3;
+        const int words = (int)orbits * kUnstageRow;
+        for (int q = threadIdx.x; q < words; q += blockDim.x) tile[q / kUnstageRow][q % kUnstageRow] = src[q];
+        __syncthreads();
+        const int32_t s0 = b * kStageBlock;
+        const int rows = min(kStageBlock, nt - s0);
+        const int row_words = (int)orbits * 3;
+        double2* dst = reinterpret_cast<double2*>(out);
+        for (int q = threadIdx.x; q < rows * row_words; q += blockDim.x) {
+            const int r = q / row_words, rem = q % row_words;
+            const int o = rem / 3, part = rem % 3;
+            dst[(((int64_t)(s0 + r)) * n + o0 + o) * 3 + part] = tile[o][r * 3 + part];
+        }
+        __syncthreads();
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Cost ordering: a counting sort of the orbits by a bucket of their estimated
@@ -685,8 +733,8 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         return fail(DRIFT_ERR_UNSUPPORTED, "schedule must be PLAIN, PERSISTENT or TUNED");
     if (opt.ordering < DRIFT_ORDER_AUTO || opt.ordering > DRIFT_ORDER_COST)
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "ordering must be AUTO, INDEX or COST");
-    if (opt.reserved != 0)
-        return fail(DRIFT_ERR_INVALID_ARGUMENT, "drift_launch_options.reserved must be 0");
+    if (opt.staging < DRIFT_STAGING_AUTO || opt.staging > DRIFT_STAGING_STAGED)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "staging must be AUTO, DIRECT or STAGED");
 
     PlainArgs a{};
     a.state0 = d_state0;
@@ -726,6 +774,7 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         opt.schedule = e ? e->options.schedule : DRIFT_SCHEDULE_PERSISTENT;
         opt.block_size = e ? e->options.block_size : 0;
         opt.ordering = e ? e->options.ordering : DRIFT_ORDER_AUTO;
+        if (opt.staging == DRIFT_STAGING_AUTO && e) opt.staging = e->options.staging;
         block = opt.block_size ? opt.block_size : kDefaultBlock;
     }
 
@@ -768,9 +817,38 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         if ((rc = build_cost_order(s, a, opt.precision, st)) != DRIFT_OK) return rc;
         a.order = static_cast<const uint32_t*>(s->order_slots.ptr);
     }
+    const bool staged = opt.staging == DRIFT_STAGING_STAGED;
+    const size_t stage_blocks = (nt + kStageBlock - 1) / kStageBlock;
+    if (staged) {
+        const size_t bytes = n * stage_blocks * kStageBlock * DIM * sizeof(double);
+        if (capturing) {
+            if (s->staging.bytes < bytes)
+                return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                            "stream capture: run one eager staged launch of at least this size on "
+                            "this session first (it allocates the staging buffer)");
+        } else {
+            if ((rc = ensure_buffer(s->staging, bytes)) != DRIFT_OK) return rc;
+            // the staging buffer is per session: wait for the last launch that used it
+            if (!s->staging_done)
+                DRIFT_HIP_TRY(hipEventCreateWithFlags(&s->staging_done, hipEventDisableTiming));
+            else
+                DRIFT_HIP_TRY(hipStreamWaitEvent(st, s->staging_done, 0));
+        }
+        a.out = static_cast<double*>(s->staging.ptr);
+        a.staged = 1;
+    }
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, st, a);
     DRIFT_HIP_TRY(hipGetLastError());
     if (cost_order) DRIFT_HIP_TRY(hipEventRecord(s->order_done, st));
+    if (staged) {
+        const int64_t tiles = ((int64_t)n + kUnstageOrbits - 1) / kUnstageOrbits;
+        if (tiles > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");
+        const unsigned gy = (unsigned)std::min<size_t>(stage_blocks, 65535);
+        hipLaunchKernelGGL(unstage_kernel, dim3((unsigned)tiles, gy), dim3(256), 0, st,
+                           (const double*)s->staging.ptr, d_out, (int64_t)n, (int32_t)nt);
+        DRIFT_HIP_TRY(hipGetLastError());
+        if (!capturing) DRIFT_HIP_TRY(hipEventRecord(s->staging_done, st));
+    }
     return DRIFT_OK;
 }
 
@@ -869,9 +947,11 @@ int drift_session_destroy(drift_session* s) {
         (void)hipFree(t.d_far);
     }
     for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats, &s->order_keys,
-                            &s->order_slots, &s->order_bins, &s->tune_out, &s->tune_stats})
+                            &s->order_slots, &s->order_bins, &s->tune_out, &s->tune_stats,
+                            &s->staging})
         if (b->ptr) (void)hipFree(b->ptr);
     if (s->order_done) (void)hipEventDestroy(s->order_done);
+    if (s->staging_done) (void)hipEventDestroy(s->staging_done);
     if (s->queues) (void)hipFree(s->queues);
     for (auto& r : s->ragged) {
         if (r.desc.ptr) (void)hipFree(r.desc.ptr);
@@ -992,6 +1072,32 @@ int drift_tune_device(drift_session* s, int32_t method, const double* d_state0,
                     have = true;
                 }
             }
+        }
+    }
+    // then the fastest configuration with staged output
+    if (rc == DRIFT_OK && have) {
+        drift_launch_options o = chosen;
+        o.staging = DRIFT_STAGING_STAGED;
+        float ms = 0.0f;
+        if ((rc = trial(o)) == DRIFT_OK) {  // untimed: allocates the staging buffer
+            if (hipEventRecord(e0, st) != hipSuccess) rc = DRIFT_ERR_DEVICE;
+            if (rc == DRIFT_OK) rc = trial(o);
+            if (rc == DRIFT_OK && hipEventRecord(e1, st) != hipSuccess) rc = DRIFT_ERR_DEVICE;
+            if (rc == DRIFT_OK && hipEventSynchronize(e1) != hipSuccess) rc = DRIFT_ERR_DEVICE;
+            if (rc == DRIFT_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                rc = DRIFT_ERR_DEVICE;
+        }
+        if (rc != DRIFT_OK) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (rc == DRIFT_ERR_DEVICE) set_last_error("autotuner: event timing failed");
+            return rc;
+        }
+        if (ms < chosen_ms) {
+            chosen = o;
+            chosen_ms = ms;
+        } else {
+            chosen.staging = DRIFT_STAGING_DIRECT;
         }
     }
     (void)hipEventDestroy(e0);

@@ -56,6 +56,10 @@ struct drift_session {
     // autotuner: chosen configurations and the scratch output of its trials
     std::vector<drift::TunedEntry> tuned;
     drift::DeviceBuffer tune_out, tune_stats;
+    // output staging (DRIFT_STAGING_STAGED): orbit-blocked sample buffer;
+    // staging_done marks the end of the last launch that used it
+    drift::DeviceBuffer staging;
+    hipEvent_t staging_done = nullptr;
 };
 
 #define DRIFT_HIP_TRY(expr)                                                              \

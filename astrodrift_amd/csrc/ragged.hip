@@ -487,6 +487,11 @@ extern "C" int drift_ragged_device(drift_session* session, const drift_job* jobs
         return fail(DRIFT_ERR_UNSUPPORTED, "unknown precision mode");
     if (opt.block_size && (opt.block_size < 64 || opt.block_size > 256 || opt.block_size % 64))
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
+    if (opt.staging == DRIFT_STAGING_STAGED)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "ragged launches write every job's output directly (staging is DIRECT)");
+    if (opt.staging < DRIFT_STAGING_AUTO || opt.staging > DRIFT_STAGING_STAGED)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "staging must be AUTO, DIRECT or STAGED");
     DRIFT_HIP_TRY(hipSetDevice(session->device));
     std::vector<PotParams> pots;
     std::vector<const void*> t64, t32;

@@ -10,8 +10,10 @@ rtol = atol = 1e-12, 1,001 dense outputs over t = 0..50, FP64; the
 Prints one JSON line: accepted particle-steps/s of the chosen schedule, the
 plain (one lane per orbit) figure beside it, the roofline of the integrator
 kernel (algorithmic FP64 FLOP/s from the per-orbit step statistics, over the
-kernel's HIP-event duration), the p50/p99 latency of the serving loop under a
-Poisson stream of tidal-stream requests, and the CPU oracle timed on a strided
+kernel's HIP-event duration), the same workload at rtol = 1e-7 in FP64 and in
+the mixed mode (FP32 stage forces) with its energy and position errors, the
+p50/p99 latency of the serving loop under a Poisson stream of tidal-stream
+requests, a ragged batch of mixed jobs, and the CPU oracle timed on a strided
 sample of the same orbits.
 """
 
@@ -51,6 +53,8 @@ def parse_args():
     p.add_argument("--serving-rate", type=float, default=8.0, help="requests per second")
     p.add_argument("--ragged-jobs", type=int, default=2048,
                    help="jobs of the ragged mix run as one batch (0 = skip)")
+    p.add_argument("--no-precision", action="store_true",
+                   help="skip the rtol = 1e-7 FP64 / mixed-precision leg")
     return p.parse_args()
 
 
@@ -116,6 +120,92 @@ def ragged_leg(njobs: int):
         "dop853_jobs": sum(j.method == "DOP853" for j in jobs),
         "accepted": acc, "rejected": rej, "flagged_orbits": bad,
         "seconds": round(wall, 3), "value": acc / wall, "unit": UNIT,
+    }
+
+
+def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
+    """The 1M-orbit Milky-Way DOP853 workload at rtol = atol = 1e-7: FP64 and
+    the mixed mode (FP32 stage forces, FP64 state / error norm / step
+    control), each with the plain and the persistent (cost-ordered) schedule.
+    Reports accepted particle-steps/s per run, the mixed persistent run over
+    the FP64 plain one, |dE/E| at t = 50 for every orbit (closed-form
+    potential), and position error at t = 50 against an FP64 rtol = 1e-13 run
+    of a strided subset."""
+    import numpy as np
+    import torch
+
+    from astrodrift_amd import _native, workloads
+
+    dev = torch.device("cuda:0")
+    w = workloads.mw_disk(n=n, rtol=1e-7, atol=1e-7)
+    s0 = torch.from_numpy(w.state0).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    out = torch.empty((w.nt, w.n, 6), dtype=torch.float64, device=dev)
+    stats = torch.zeros((w.n, 4), dtype=torch.int32, device=dev)
+    lt = math.log(1e-7)
+    runs, finals = {}, {}
+    for prec_name, prec in (("fp64", _native.PRECISION_FP64), ("mixed", _native.PRECISION_MIXED)):
+        for sched_name, sched, staging in (
+                ("plain", _native.SCHEDULE_PLAIN, _native.STAGING_DIRECT),
+                ("persistent", _native.SCHEDULE_PERSISTENT, _native.STAGING_DIRECT),
+                ("persistent_staged", _native.SCHEDULE_PERSISTENT, _native.STAGING_STAGED)):
+            def launch():
+                _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(),
+                                      out.data_ptr(), w.n, w.nt, lt, lt, w.potential,
+                                      stats.data_ptr(), precision=prec, schedule=sched,
+                                      staging=staging, stream=stream.cuda_stream)
+
+            for _ in range(warmup):
+                launch()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                launch()
+            torch.cuda.synchronize()
+            sec = (time.perf_counter() - t0) / steps
+            st = stats.cpu().numpy()
+            acc = int(st[:, 0].astype(np.int64).sum())
+            key = f"{prec_name}_{sched_name}"
+            runs[key] = {"ms": round(sec * 1e3, 3), "accepted": acc,
+                         "value": acc / sec, "bad_status": int(np.count_nonzero(st[:, 3]))}
+            if sched_name == "persistent_staged":
+                finals[prec_name] = out[-1].cpu().numpy()
+            print(f"[bench] precision {key}: {runs[key]}", file=sys.stderr, flush=True)
+    del out
+    torch.cuda.empty_cache()
+    e0 = workloads.mw_energy(w.potential, w.state0)
+    energy = {}
+    for k, fin in finals.items():
+        de = np.abs((workloads.mw_energy(w.potential, fin) - e0) / e0)
+        energy[k] = {"median": float(np.median(de)), "p99": float(np.quantile(de, 0.99)),
+                     "max": float(de.max())}
+    # reference trajectories of a strided subset at rtol = atol = 1e-13
+    idx = np.arange(0, w.n, max(1, w.n // subset))
+    ss = torch.from_numpy(np.ascontiguousarray(w.state0[idx])).to(dev)
+    so = torch.empty((w.nt, len(idx), 6), dtype=torch.float64, device=dev)
+    _native.launch_device(_native.METHOD_DOP853, ss.data_ptr(), tt.data_ptr(), so.data_ptr(),
+                          len(idx), w.nt, math.log(1e-13), math.log(1e-13), w.potential,
+                          schedule=_native.SCHEDULE_PERSISTENT, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    truth = so[-1].cpu().numpy()
+    position = {}
+    for k, fin in finals.items():
+        err = np.linalg.norm(fin[idx, :3] - truth[:, :3], axis=1) / \
+            np.linalg.norm(truth[:, :3], axis=1)
+        position[k] = {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)),
+                       "max": float(err.max())}
+    best_mixed = max(runs["mixed_persistent"]["value"], runs["mixed_persistent_staged"]["value"])
+    best_fp64 = max(runs["fp64_persistent"]["value"], runs["fp64_persistent_staged"]["value"])
+    return {
+        "workload": f"mw_disk_dop853: {w.n} orbits, DOP853 rtol=atol=1e-7, {w.nt} outputs over "
+                    "t=0..50; mixed = FP32 stage forces, FP64 state/error norm/step control; "
+                    "staged = orbit-blocked samples + tiled transpose to the time-major output",
+        "runs": runs,
+        "mixed_persistent_over_fp64_plain": best_mixed / runs["fp64_plain"]["value"],
+        "mixed_persistent_over_fp64_persistent": best_mixed / best_fp64,
+        "energy_rel_err_t50": energy,
+        "position_rel_err_t50_vs_fp64_rtol1e-13": position,
+        "position_subset": len(idx),
     }
 
 
@@ -273,18 +363,17 @@ def main() -> int:
             "traffic_source": str(TRAFFIC_PROFILE.relative_to(REPO)),
         },
     }
+    del out  # release the 50 GB batch output before the other legs
+    torch.cuda.empty_cache()
+    if not args.no_precision:
+        line["precision"] = precision_leg(args.steps, args.warmup, args.n, stream)
     if args.serving_seconds > 0:
         from astrodrift_amd import loadgen
 
-        del out  # release the 50 GB batch output before serving
-        torch.cuda.empty_cache()
         srv = loadgen.run_poisson(rate_hz=args.serving_rate, duration_s=args.serving_seconds)
         line["serving"] = srv
         print(f"[bench] serving: {srv}", file=sys.stderr, flush=True)
     if args.ragged_jobs > 0:
-        if args.serving_seconds <= 0:
-            del out
-            torch.cuda.empty_cache()
         line["ragged"] = ragged_leg(args.ragged_jobs)
         print(f"[bench] ragged: {line['ragged']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:

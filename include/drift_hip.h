@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define DRIFT_ABI_VERSION 2
+#define DRIFT_ABI_VERSION 3
 
 /* Status codes.  They map onto the reference's error taxonomy
  * (DispatchError, src/dispatch/mod.rs:36-79; IntegrationError,
@@ -107,8 +107,26 @@ typedef struct drift_launch_options {
                               estimate never drops below 1 retries forever at the
                               step floor. */
     int32_t ordering;   /* drift_ordering: the order in which lanes take orbits */
-    int32_t reserved;   /* must be 0 */
+    int32_t staging;    /* drift_output_staging: how samples reach the time-major output */
 } drift_launch_options;
+
+/* How the integrator writes its samples.  The output is time-major: sample s
+ * of orbit i is the 48-byte record at ((s * n) + i) * 6, and a lane writes its
+ * orbit's records one at a time as the orbit reaches them.  Lanes that carry
+ * neighbouring orbits rarely reach a sample together (the cost ordering and
+ * the persistent queue pair unrelated orbits), so DIRECT writes land as
+ * scattered partial lines.  STAGED writes each orbit's samples in blocks of
+ * 16 consecutive samples to a session scratch buffer (whole lines, written in
+ * order) and a tiled transpose then writes the time-major output with full
+ * lines; it pays one more pass over the output and wins where samples are
+ * dense against steps (low tolerances).  Outputs are bit-identical. */
+typedef enum drift_output_staging {
+    DRIFT_STAGING_AUTO = 0,   /* the autotuner's choice for TUNED launches, else DIRECT */
+    DRIFT_STAGING_DIRECT = 1,
+    DRIFT_STAGING_STAGED = 2  /* needs n * ceil(nt / 16) * 16 * 48 bytes of session scratch
+                                 (allocated by an eager launch; refused under capture
+                                 until then) */
+} drift_output_staging;
 
 /* Orbit order of a launch.  Outputs never depend on it (orbits are
  * independent); it only changes how evenly the lanes of a wavefront are
@@ -192,9 +210,10 @@ int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
 /* Autotuner (no counterpart in the reference).  Times every candidate
  * {PLAIN, PERSISTENT} x block {64, 128, 256} x ordering {INDEX, COST} on the
  * launch described by the arguments (PLAIN only while n is below twice the
- * device's resident lanes: a prefix hides the plain schedule's tail), each
- * integrated over the first prefix_samples output times only
- * (0 = max(2, nt / 20)) so that every orbit takes part, and
+ * device's resident lanes: a prefix hides the plain schedule's tail), then
+ * the fastest with STAGED output, each integrated over the first
+ * prefix_samples output times only (0 = max(2, nt / 20)) so that every orbit
+ * takes part, and
  * returns the fastest in *best (precision and max_attempts as given in
  * *best on entry, or 0) and its time in *best_ms (nullable).  The choice is
  * stored in the session under (method, potential kind, precision, log2 n) for

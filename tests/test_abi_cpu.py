@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_layouts():
     lib = _native.load_library()
-    assert lib.drift_get_abi_version() == _native.ABI_VERSION == 2
+    assert lib.drift_get_abi_version() == _native.ABI_VERSION == 3
     assert ctypes.sizeof(_native.CPotential) == 8 + 10 * 8
     assert ctypes.sizeof(_native.COrbitStats) == 16
     assert ctypes.sizeof(_native.CLaunchOptions) == 24

@@ -114,7 +114,7 @@ def test_bad_ordering_options_are_rejected(gpu):
         ctypes.c_void_p(out.data_ptr()), w.n, w.nt, -23.0, -23.0, _native.INITIAL_STEP_SENTINEL,
         None, None, ctypes.byref(opts), None)
     assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT
-    assert "reserved" in _native.last_error()
+    assert "staging" in _native.last_error()
 
 
 def test_autotuner_choice_is_valid_and_tuned_launches_match(gpu):
