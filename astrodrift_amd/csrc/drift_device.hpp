@@ -49,6 +49,12 @@ constexpr int TABLE_DEG = 11;
 constexpr int TABLE_NCOEF = TABLE_DEG + 1;
 constexpr int TABLE_SEGMENTS = (TABLE_E_MAX - TABLE_E_MIN) << TABLE_SUBBITS;
 constexpr int TABLE_DOUBLES = TABLE_SEGMENTS * TABLE_NCOEF;
+// The mixed-precision mode's float copy: the same segments, each with its own
+// degree-5 interpolant (interpolation error below 4e-9, under float rounding),
+// stored in 8-float rows so a row is two 16-byte loads.
+constexpr int TABLE32_NCOEF = 6;
+constexpr int TABLE32_STRIDE = 8;
+constexpr int TABLE32_FLOATS = TABLE_SEGMENTS * TABLE32_STRIDE;
 constexpr int FAR_E_MIN = -1074;
 constexpr int FAR_E_MAX = 1024;
 constexpr int FAR_SEGMENTS = (FAR_E_MAX - FAR_E_MIN) << TABLE_SUBBITS;
@@ -133,11 +139,11 @@ DRIFT_HD DRIFT_FI bool table_locate(double r2, int& seg, double& s) {
     return table_locate_bits(r2, TABLE_E_MIN + 1023, (uint32_t)TABLE_SEGMENTS, seg, s);
 }
 
-template <class T>
+template <class T, int NCOEF = TABLE_NCOEF>
 DRIFT_HD DRIFT_FI T horner_table(const T* c, T s) {
-    T acc = c[TABLE_DEG];
+    T acc = c[NCOEF - 1];
 #pragma unroll
-    for (int k = TABLE_DEG - 1; k >= 0; --k) acc = acc * s + c[k];
+    for (int k = NCOEF - 2; k >= 0; --k) acc = acc * s + c[k];
     return acc;
 }
 

@@ -70,7 +70,7 @@ struct PlainArgs {
     double dt_one_init;   // DOPR54 only
     double bound2[6];     // DOPR54 only
     PotParams pot;
-    const void* table;    // MW: TABLE_DOUBLES coefficients (double or float)
+    const void* table;    // MW: TABLE_DOUBLES doubles, or TABLE32_FLOATS floats (mixed)
     unsigned long long* queue;  // persistent schedule: next unclaimed orbit slot
     const uint32_t* order;      // slot -> orbit (NULL = identity)
     int32_t staged;             // out is the orbit-blocked staging buffer
@@ -107,7 +107,8 @@ template <int KIND, class FT>
 __device__ __forceinline__ const FT* stage_table(const void* gtable, FT* lds) {
     if constexpr (KIND == POT_MW) {
         const FT* g = static_cast<const FT*>(gtable);
-        for (int k = threadIdx.x; k < TABLE_DOUBLES; k += blockDim.x) lds[k] = g[k];
+        constexpr int words = sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS;
+        for (int k = threadIdx.x; k < words; k += blockDim.x) lds[k] = g[k];
         __syncthreads();
         return lds;
     } else {
@@ -130,7 +131,7 @@ constexpr int kDefaultBlock = 256;
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_plain_kernel(PlainArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= a.n) return;
@@ -158,7 +159,7 @@ __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_plain_ke
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) dopr54_plain_kernel(PlainArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= a.n) return;
@@ -215,7 +216,7 @@ __device__ __forceinline__ bool claim_orbit(bool need, unsigned long long* queue
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_persistent_kernel(PlainArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
     const auto f = ForceFor<KIND, FT>::make(a.pot, table);
     const double t0 = a.times[0];
@@ -250,7 +251,7 @@ __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_persiste
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) dopr54_persistent_kernel(PlainArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
     const auto f = ForceFor<KIND, FT>::make(a.pot, table);
     Dopr54Job J;
@@ -358,7 +359,7 @@ __global__ void order_reset_kernel(uint32_t* bins) {
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) order_key_kernel(PlainArgs a, uint16_t* keys,
                                                                    uint32_t* bins) {
-    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     __shared__ uint32_t local[kOrderBins];
     for (int k = threadIdx.x; k < kOrderBins; k += blockDim.x) local[k] = 0u;
     const FT* table = stage_table<KIND, FT>(a.table, lds);  // ends with a barrier
@@ -440,7 +441,7 @@ struct ForceArgs {
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) force_kernel(ForceArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? TABLE_DOUBLES : 1];
+    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -556,16 +557,16 @@ int resolve_potential(drift_session* s, const drift_potential* potential, PotPar
         std::vector<double> h64(TABLE_DOUBLES), far(FAR_DOUBLES);
         build_mw_table(p, h64.data());
         build_mw_far_table(p, far.data());
-        std::vector<float> h32(TABLE_DOUBLES);
-        for (int k = 0; k < TABLE_DOUBLES; ++k) h32[k] = (float)h64[k];
+        std::vector<float> h32(TABLE32_FLOATS);
+        build_mw_table_f32(p, h32.data());
         hipError_t err = hipMalloc(&e.d_table64, TABLE_DOUBLES * sizeof(double));
-        if (err == hipSuccess) err = hipMalloc(&e.d_table32, TABLE_DOUBLES * sizeof(float));
+        if (err == hipSuccess) err = hipMalloc(&e.d_table32, TABLE32_FLOATS * sizeof(float));
         if (err == hipSuccess) err = hipMalloc(&e.d_far, FAR_DOUBLES * sizeof(double));
         if (err == hipSuccess)
             err = hipMemcpy(e.d_table64, h64.data(), TABLE_DOUBLES * sizeof(double),
                             hipMemcpyHostToDevice);
         if (err == hipSuccess)
-            err = hipMemcpy(e.d_table32, h32.data(), TABLE_DOUBLES * sizeof(float),
+            err = hipMemcpy(e.d_table32, h32.data(), TABLE32_FLOATS * sizeof(float),
                             hipMemcpyHostToDevice);
         if (err == hipSuccess)
             err = hipMemcpy(e.d_far, far.data(), FAR_DOUBLES * sizeof(double),

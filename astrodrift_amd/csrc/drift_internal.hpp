@@ -88,6 +88,9 @@ int resolve_potential(drift_session* s, const drift_potential* potential, PotPar
 // Milky-Way spherical table (spherical_table.cpp).
 void build_mw_params(const drift_potential& p, PotParams& out);
 void build_mw_table(const drift_potential& p, double* coeffs);
+// The mixed mode's float table: TABLE32_FLOATS values, TABLE32_NCOEF per
+// TABLE32_STRIDE-float row.
+void build_mw_table_f32(const drift_potential& p, float* coeffs);
 double mw_table_max_rel_error(const drift_potential& p, const double* coeffs, int samples_per_seg);
 void build_mw_far_table(const drift_potential& p, double* coeffs);
 double mw_far_table_max_rel_error(const drift_potential& p, const double* coeffs,

@@ -57,7 +57,8 @@ struct PlummerForce {
 
 // Milky-Way model: Miyamoto-Nagai disk in closed form plus the tabulated
 // spherical part (bulge + NFW halo), see spherical_table.cpp.  `table` points
-// at TABLE_DOUBLES coefficients (in LDS inside the kernels).
+// at the LDS table: TABLE_DOUBLES double coefficients, or for FT = float the
+// TABLE32_FLOATS of the degree-5 float table.
 template <class FT>
 struct MwForce {
     const PotParams* p;  // far-table pointer (outside the LDS table's range)
@@ -87,7 +88,11 @@ struct MwForce {
         // table, so the common path carries no divergent branch
         const uint32_t last_row = TABLE_SEGMENTS - 1;
         const uint32_t row = (uint32_t)seg < last_row ? (uint32_t)seg : last_row;
-        FT S = horner_table<FT>(table + row * TABLE_NCOEF, FT(s));
+        FT S;
+        if constexpr (sizeof(FT) == 8)
+            S = horner_table<FT>(table + row * TABLE_NCOEF, FT(s));
+        else
+            S = horner_table<FT, TABLE32_NCOEF>(table + row * TABLE32_STRIDE, FT(s));
         if (!in_lds) S = FT(far_spherical(*p, r2));
         // both parts are multiples of (x, y) in the plane: a = (f - S) (x, y)
         const FT g = f - S;

@@ -105,13 +105,14 @@ void segment_bounds(int seg, int e_min, ld& lo, ld& hi, int& e) {
     hi = lo + w;
 }
 
-// Chebyshev interpolants of S on the segments of [2^e_min, 2^e_max), as
-// monomial coefficients in the segment's local variable s in [-1, 1].
-void build_table(const drift_potential& p, int e_min, int e_max, double* coeffs) {
+// Chebyshev interpolants of S with N coefficients on the segments of
+// [2^e_min, 2^e_max), as monomial coefficients in the segment's local variable
+// s in [-1, 1], written through store(segment, k, value).
+template <int N, class Store>
+void build_table_n(const drift_potential& p, int e_min, int e_max, Store store) {
     const MwSpherical f = make_spherical(p);
-    const int N = TABLE_NCOEF;
     const ld pi = 3.14159265358979323846264338327950288L;
-    ld node[TABLE_NCOEF], basis[TABLE_NCOEF][TABLE_NCOEF];
+    ld node[N], basis[N][N];
     for (int j = 0; j < N; ++j) node[j] = cosl(pi * (j + 0.5L) / N);
     for (int k = 0; k < N; ++k)
         for (int j = 0; j < N; ++j) basis[k][j] = cosl(k * pi * (j + 0.5L) / N);
@@ -123,7 +124,7 @@ void build_table(const drift_potential& p, int e_min, int e_max, double* coeffs)
         const ld scale = ldexpl(1.0L, e);
         const ld centre = 0.5L * (lo + hi), half = 0.5L * (hi - lo);
         // Chebyshev interpolation at the N first-kind nodes.
-        ld fv[TABLE_NCOEF], cheb[TABLE_NCOEF];
+        ld fv[N], cheb[N];
         for (int j = 0; j < N; ++j) fv[j] = f((centre + half * node[j]) * scale);
         for (int k = 0; k < N; ++k) {
             ld acc = 0;
@@ -132,8 +133,8 @@ void build_table(const drift_potential& p, int e_min, int e_max, double* coeffs)
         }
         cheb[0] *= 0.5L;
         // Convert sum_k cheb[k] T_k(s) to monomials in s.
-        ld mono[TABLE_NCOEF] = {};
-        ld tkm1[TABLE_NCOEF] = {}, tk[TABLE_NCOEF] = {}, tkp1[TABLE_NCOEF];
+        ld mono[N] = {};
+        ld tkm1[N] = {}, tk[N] = {}, tkp1[N];
         tkm1[0] = 1;  // T_0
         tk[1] = 1;    // T_1
         mono[0] += cheb[0];
@@ -146,8 +147,14 @@ void build_table(const drift_potential& p, int e_min, int e_max, double* coeffs)
                 tk[i] = tkp1[i];
             }
         }
-        for (int i = 0; i < N; ++i) coeffs[seg * N + i] = (double)mono[i];
+        for (int i = 0; i < N; ++i) store(seg, i, mono[i]);
     }
+}
+
+void build_table(const drift_potential& p, int e_min, int e_max, double* coeffs) {
+    build_table_n<TABLE_NCOEF>(p, e_min, e_max, [coeffs](int seg, int k, ld v) {
+        coeffs[seg * TABLE_NCOEF + k] = (double)v;
+    });
 }
 
 }  // namespace
@@ -164,6 +171,13 @@ void build_mw_params(const drift_potential& p, PotParams& out) {
 
 void build_mw_table(const drift_potential& p, double* coeffs) {
     build_table(p, TABLE_E_MIN, TABLE_E_MAX, coeffs);
+}
+
+void build_mw_table_f32(const drift_potential& p, float* coeffs) {
+    for (int k = 0; k < TABLE32_FLOATS; ++k) coeffs[k] = 0.0f;
+    build_table_n<TABLE32_NCOEF>(p, TABLE_E_MIN, TABLE_E_MAX, [coeffs](int seg, int k, ld v) {
+        coeffs[seg * TABLE32_STRIDE + k] = (float)v;
+    });
 }
 
 void build_mw_far_table(const drift_potential& p, double* coeffs) {
