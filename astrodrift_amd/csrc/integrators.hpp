@@ -421,6 +421,47 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
             // h dense_k(F), the velocity halves sum_j d_kj F_j.
             L.dense++;
             double r1[DIM], r2[DIM], r3[DIM], r4[DIM], r5[DIM], r6[DIM], r7[DIM], r8[DIM];
+            // the three extra stages (dop853.rs:703-742)
+            rkn853_stage<14, 15>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+            rkn853_stage<15, 16>(J, y0, h, hh, Fs, Lc1, Lc0, f);
+            rkn853_stage<16, 0>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+            // rcont5..8 after the stages, one coefficient group at a time (the
+            // groups' scalar loads then fit the scalar registers); the forces
+            // die as the sums consume them
+            {
+                const Dop853Coef C = dop853_coef(J.coef, Fs[16][2]);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    r5[c] = rkn853::dense4pos_pre(Fs, c, C) + rkn853::dense4pos_post(Fs, c, C);
+                    r5[3 + c] = rkn853::dense4vel_pre(Fs, c, C) + rkn853::dense4vel_post(Fs, c, C);
+                }
+            }
+            {
+                const Dop853Coef C = dop853_coef(J.coef, r5);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    r6[c] = rkn853::dense5pos_pre(Fs, c, C) + rkn853::dense5pos_post(Fs, c, C);
+                    r6[3 + c] = rkn853::dense5vel_pre(Fs, c, C) + rkn853::dense5vel_post(Fs, c, C);
+                }
+            }
+            {
+                const Dop853Coef C = dop853_coef(J.coef, r6);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    r7[c] = rkn853::dense6pos_pre(Fs, c, C) + rkn853::dense6pos_post(Fs, c, C);
+                    r7[3 + c] = rkn853::dense6vel_pre(Fs, c, C) + rkn853::dense6vel_post(Fs, c, C);
+                }
+            }
+            {
+                const Dop853Coef C = dop853_coef(J.coef, r7);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    r8[c] = rkn853::dense7pos_pre(Fs, c, C) + rkn853::dense7pos_post(Fs, c, C);
+                    r8[3 + c] = rkn853::dense7vel_pre(Fs, c, C) + rkn853::dense7vel_post(Fs, c, C);
+                }
+            }
+            // rcont1..4 last: their inputs stay live anyway, the results only
+            // from here on
 #pragma unroll
             for (int i = 0; i < DIM; i++) {
                 // k1 = (v0, F1), k4 (the new derivative) = (v_new, F13)
@@ -432,64 +473,6 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
                 double bspl = h * k1i - ydiff;
                 r3[i] = bspl;
                 r4[i] = ydiff - h * k4i - bspl;
-            }
-            // one coefficient group (16 values) at a time: the groups'
-            // scalar loads then fit the scalar registers
-            {
-                const Dop853Coef C = dop853_coef(J.coef, Fs[13][2]);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r5[c] = rkn853::dense4pos_pre(Fs, c, C);
-                    r5[3 + c] = rkn853::dense4vel_pre(Fs, c, C);
-                }
-            }
-            {
-                const Dop853Coef C = dop853_coef(J.coef, r5);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r6[c] = rkn853::dense5pos_pre(Fs, c, C);
-                    r6[3 + c] = rkn853::dense5vel_pre(Fs, c, C);
-                }
-            }
-            {
-                const Dop853Coef C = dop853_coef(J.coef, r6);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r7[c] = rkn853::dense6pos_pre(Fs, c, C);
-                    r7[3 + c] = rkn853::dense6vel_pre(Fs, c, C);
-                }
-            }
-            {
-                const Dop853Coef C = dop853_coef(J.coef, r7);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r8[c] = rkn853::dense7pos_pre(Fs, c, C);
-                    r8[3 + c] = rkn853::dense7vel_pre(Fs, c, C);
-                }
-            }
-            // the three extra stages (dop853.rs:703-742)
-            rkn853_stage<14, 15>(J, y0, h, hh, Fs, Lc0, Lc1, f);
-            rkn853_stage<15, 16>(J, y0, h, hh, Fs, Lc1, Lc0, f);
-            rkn853_stage<16, 0>(J, y0, h, hh, Fs, Lc0, Lc1, f);
-            {
-                const Dop853Coef C = dop853_coef(J.coef, Fs[16][2]);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r5[c] += rkn853::dense4pos_post(Fs, c, C);
-                    r6[c] += rkn853::dense5pos_post(Fs, c, C);
-                    r5[3 + c] += rkn853::dense4vel_post(Fs, c, C);
-                    r6[3 + c] += rkn853::dense5vel_post(Fs, c, C);
-                }
-            }
-            {
-                const Dop853Coef C = dop853_coef(J.coef, r6);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r7[c] += rkn853::dense6pos_post(Fs, c, C);
-                    r8[c] += rkn853::dense7pos_post(Fs, c, C);
-                    r7[3 + c] += rkn853::dense6vel_post(Fs, c, C);
-                    r8[3 + c] += rkn853::dense7vel_post(Fs, c, C);
-                }
             }
             const double inv_h = rcp_fast(h);
             double t_s = L.t_next;
