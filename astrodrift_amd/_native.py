@@ -6,10 +6,11 @@ every integration raises :class:`NativeLibraryError`; there is no fallback.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import math
 import threading
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from pathlib import Path
 
 import numpy as np
@@ -32,6 +33,7 @@ POTENTIAL_GALPY_KEPLER = 0
 POTENTIAL_KEPLER = 1
 POTENTIAL_PLUMMER = 2
 POTENTIAL_MW = 3
+POTENTIAL_NONE = 4  # no static force (moving origins only, or free motion)
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -47,7 +49,7 @@ STAGING_AUTO = 0
 STAGING_DIRECT = 1
 STAGING_STAGED = 2
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 STATUS_CODE_MASK = 0xFF
 STATUS_NONFINITE = 0x100
@@ -72,7 +74,7 @@ class DriftNativeError(RuntimeError):
 class CPotential(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("origins", ctypes.c_int32),
         ("amp", ctypes.c_double),
         ("b", ctypes.c_double),
         ("bulge_amp", ctypes.c_double),
@@ -149,9 +151,83 @@ def _lower_gamma(a: float, x: float) -> float:
     return math.gamma(a) - math.exp(-x + a * math.log(x)) * h
 
 
+ORIGIN_COEFS = 18
+
+
+@dataclass(frozen=True, eq=False)
+class Origins:
+    """Moving origins (the reference's CustomOrigin, shared/src/potential.rs:
+    264-356): origin i is a point mass (b = 0) or Plummer sphere of amplitude
+    params[i, 0] and radius params[i, 1] whose centre follows the quintic
+    tracks[i, s] (x, y, z coefficients, tau^5 .. tau^0) on segment s of
+    n_segments equal segments of [t_start, t_end], at local time tau in [0, 1).
+    """
+
+    tracks: np.ndarray  # (n_origins, n_segments, 18)
+    params: np.ndarray  # (n_origins, 2): amp, b
+    t_start: float
+    t_end: float
+
+    def __post_init__(self):
+        tracks = np.ascontiguousarray(self.tracks, dtype=np.float64)
+        params = np.ascontiguousarray(self.params, dtype=np.float64).reshape(-1, 2)
+        if tracks.ndim != 3 or tracks.shape[2] != ORIGIN_COEFS:
+            raise ValueError("tracks must have shape (n_origins, n_segments, 18)")
+        if tracks.shape[0] != params.shape[0]:
+            raise ValueError("tracks and params must describe the same origins")
+        object.__setattr__(self, "tracks", tracks)
+        object.__setattr__(self, "params", params)
+
+    @property
+    def n_origins(self) -> int:
+        return self.tracks.shape[0]
+
+    @property
+    def n_segments(self) -> int:
+        return self.tracks.shape[1]
+
+    @staticmethod
+    def hermite(states, accels, times, params) -> "Origins":
+        """Tracks of sampled orbits (drift_origin_tracks_hermite, host only):
+        states (nt, n, 6) time-major as the integrators write them, accels
+        (nt, n, 3) the accelerations at those samples, times (nt,) uniformly
+        spaced; params (n, 2) or one (amp, b) pair for every orbit."""
+        lib = load_library()
+        y = np.ascontiguousarray(states, dtype=np.float64)
+        a = np.ascontiguousarray(accels, dtype=np.float64)
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        if y.ndim != 3 or y.shape[2] != 6 or a.shape != y.shape[:2] + (3,) or t.shape != y.shape[:1]:
+            raise ValueError("states (nt, n, 6), accels (nt, n, 3) and times (nt,) must agree")
+        nt, n = y.shape[0], y.shape[1]
+        tracks = np.empty((n, nt - 1, ORIGIN_COEFS))
+        t0, t1 = ctypes.c_double(), ctypes.c_double()
+        check(lib.drift_origin_tracks_hermite(_ptr(y), _ptr(a), _ptr(t), nt, n, _ptr(tracks),
+                                              ctypes.byref(t0), ctypes.byref(t1)))
+        pb = np.broadcast_to(np.asarray(params, dtype=np.float64).reshape(-1, 2), (n, 2))
+        return Origins(tracks, pb.copy(), t0.value, t1.value)
+
+    @staticmethod
+    def fixed(params, n_segments: int, t_start: float, t_end: float) -> "Origins":
+        """Origins resting at (0, 0, 0): a static point mass or Plummer
+        sphere expressed as a moving origin."""
+        pb = np.asarray(params, dtype=np.float64).reshape(-1, 2)
+        return Origins(np.zeros((pb.shape[0], n_segments, ORIGIN_COEFS)), pb, t_start, t_end)
+
+    @staticmethod
+    def concat(sets: list) -> "Origins":
+        """One set holding the origins of several sets on the same segments."""
+        first = sets[0]
+        for o in sets[1:]:
+            if (o.n_segments, o.t_start, o.t_end) != (first.n_segments, first.t_start, first.t_end):
+                raise ValueError("origin sets to combine must share their segments")
+        return Origins(np.concatenate([o.tracks for o in sets]),
+                       np.concatenate([o.params for o in sets]), first.t_start, first.t_end)
+
+
 @dataclass(frozen=True)
 class PotentialSpec:
-    """A force model the kernels evaluate (G = 1 natural units)."""
+    """A force model the kernels evaluate (G = 1 natural units): a static
+    kind plus, optionally, moving origins."""
 
     kind: int
     amp: float = 0.0
@@ -164,11 +240,12 @@ class PotentialSpec:
     disk_b: float = 0.0
     halo_amp: float = 0.0
     halo_a: float = 0.0
+    origins: Origins | None = field(default=None, compare=False)
 
-    def to_c(self) -> CPotential:
+    def to_c(self, origins_id: int = 0) -> CPotential:
         return CPotential(
             self.kind,
-            0,
+            origins_id,
             self.amp,
             self.b,
             self.bulge_amp,
@@ -185,6 +262,14 @@ class PotentialSpec:
     def galpy_kepler() -> "PotentialSpec":
         """The reference's hard-coded unit Kepler force (dopr54.rs:75-88)."""
         return PotentialSpec(POTENTIAL_GALPY_KEPLER)
+
+    @staticmethod
+    def none() -> "PotentialSpec":
+        """No static force (with moving origins, or free motion)."""
+        return PotentialSpec(POTENTIAL_NONE)
+
+    def with_origins(self, origins: Origins | None) -> "PotentialSpec":
+        return replace(self, origins=origins)
 
     @staticmethod
     def kepler(amp: float) -> "PotentialSpec":
@@ -288,7 +373,18 @@ def _declare(lib: ctypes.CDLL) -> None:
                                         c.POINTER(CLaunchOptions), c.c_void_p]
     lib.drift_force_device.restype = c.c_int
     lib.drift_force_device.argtypes = [c.c_void_p, c.POINTER(CPotential), c.c_void_p, c.c_void_p,
-                                       c.c_size_t, c.c_int32, c.c_void_p]
+                                       c.c_void_p, c.c_size_t, c.c_int32, c.c_void_p]
+    lib.drift_force.restype = c.c_int
+    lib.drift_force.argtypes = [c.c_void_p, c.POINTER(CPotential), c.c_void_p, c.c_void_p,
+                                c.c_void_p, c.c_size_t]
+    lib.drift_origins_create.restype = c.c_int
+    lib.drift_origins_create.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_int32, c.c_int32,
+                                         c.c_double, c.c_double, c.POINTER(c.c_int32)]
+    lib.drift_origins_destroy.restype = c.c_int
+    lib.drift_origins_destroy.argtypes = [c.c_void_p, c.c_int32]
+    lib.drift_origin_tracks_hermite.restype = c.c_int
+    lib.drift_origin_tracks_hermite.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_size_t,
+                                                c.c_size_t, c.c_void_p, _D, _D]
     lib.drift_mw_table_check.restype = c.c_int
     lib.drift_mw_table_check.argtypes = [c.POINTER(CPotential), c.c_int32, _D]
 
@@ -364,6 +460,26 @@ def _ptr(a: np.ndarray) -> ctypes.c_void_p:
     return ctypes.c_void_p(a.ctypes.data)
 
 
+@contextlib.contextmanager
+def _c_potential(s: Session, potential: PotentialSpec | None):
+    """The C descriptor of `potential` (None = NULL), with its moving origins
+    uploaded to the session for the duration of the block."""
+    if potential is None:
+        yield None
+        return
+    o = potential.origins
+    if o is None:
+        yield potential.to_c()
+        return
+    ident = ctypes.c_int32(0)
+    check(s.lib.drift_origins_create(s.handle, _ptr(o.tracks), _ptr(o.params), o.n_origins,
+                                     o.n_segments, o.t_start, o.t_end, ctypes.byref(ident)))
+    try:
+        yield potential.to_c(ident.value)
+    finally:
+        s.lib.drift_origins_destroy(s.handle, ident.value)
+
+
 def _as_state(state0) -> np.ndarray:
     a = np.ascontiguousarray(state0, dtype=np.float64)
     if a.ndim == 1:
@@ -397,14 +513,14 @@ def launch_host(
     n, nt = y0.shape[0], t.shape[0]
     out = np.empty((nt, n, 6), dtype=np.float64)
     stats = np.zeros(n, dtype=STATS_DTYPE)
-    pot = potential.to_c() if potential is not None else None
-    pot_ref = ctypes.byref(pot) if pot is not None else None
-    if method == METHOD_DOPR54:
-        code = s.lib.drift_galpy_dopr54(s.handle, _ptr(y0), _ptr(t), _ptr(out), n, nt,
-                                        rtol_ln, atol_ln, dt_one, pot_ref, _ptr(stats))
-    else:
-        code = s.lib.drift_galpy_dop853(s.handle, _ptr(y0), _ptr(t), _ptr(out), n, nt,
-                                        rtol_ln, atol_ln, pot_ref, _ptr(stats))
+    with _c_potential(s, potential) as pot:
+        pot_ref = ctypes.byref(pot) if pot is not None else None
+        if method == METHOD_DOPR54:
+            code = s.lib.drift_galpy_dopr54(s.handle, _ptr(y0), _ptr(t), _ptr(out), n, nt,
+                                            rtol_ln, atol_ln, dt_one, pot_ref, _ptr(stats))
+        else:
+            code = s.lib.drift_galpy_dop853(s.handle, _ptr(y0), _ptr(t), _ptr(out), n, nt,
+                                            rtol_ln, atol_ln, pot_ref, _ptr(stats))
     if code != DRIFT_OK and (code != DRIFT_ERR_INTEGRATION or raise_on_integration_error):
         check(code)
     return out, stats, code
@@ -431,19 +547,23 @@ def launch_device(
     ordering: int = ORDER_AUTO,
     staging: int = STAGING_AUTO,
 ) -> None:
-    """Device-resident launch on raw device pointers (e.g. torch data_ptr())."""
+    """Device-resident launch on raw device pointers (e.g. torch data_ptr()).
+
+    With moving origins the set is uploaded before the launch and freed after
+    it (freeing waits for the device)."""
     s = session(device)
     opts = CLaunchOptions(precision, block_size, schedule, max_attempts, ordering, staging)
-    pot = potential.to_c() if potential is not None else None
-    pot_ref = ctypes.byref(pot) if pot is not None else None
     args = (s.handle, ctypes.c_void_p(d_state0), ctypes.c_void_p(d_times), ctypes.c_void_p(d_out),
             n, nt, rtol_ln, atol_ln)
-    if method == METHOD_DOPR54:
-        code = s.lib.drift_galpy_dopr54_device(*args, dt_one, pot_ref, ctypes.c_void_p(d_stats),
-                                               ctypes.byref(opts), ctypes.c_void_p(stream))
-    else:
-        code = s.lib.drift_galpy_dop853_device(*args, pot_ref, ctypes.c_void_p(d_stats),
-                                               ctypes.byref(opts), ctypes.c_void_p(stream))
+    with _c_potential(s, potential) as pot:
+        pot_ref = ctypes.byref(pot) if pot is not None else None
+        if method == METHOD_DOPR54:
+            code = s.lib.drift_galpy_dopr54_device(*args, dt_one, pot_ref,
+                                                   ctypes.c_void_p(d_stats), ctypes.byref(opts),
+                                                   ctypes.c_void_p(stream))
+        else:
+            code = s.lib.drift_galpy_dop853_device(*args, pot_ref, ctypes.c_void_p(d_stats),
+                                                   ctypes.byref(opts), ctypes.c_void_p(stream))
     check(code)
 
 
@@ -456,11 +576,13 @@ def tune_device(method: int, d_state0: int, d_times: int, n: int, nt: int, rtol_
     s = session(device)
     best = CLaunchOptions(precision, 0, 0, 0, 0, 0)
     ms = ctypes.c_double(0.0)
-    pot = potential.to_c() if potential is not None else None
-    check(s.lib.drift_tune_device(s.handle, method, ctypes.c_void_p(d_state0),
-                                  ctypes.c_void_p(d_times), n, nt, rtol_ln, atol_ln, dt_one,
-                                  ctypes.byref(pot) if pot is not None else None, prefix_samples,
-                                  ctypes.byref(best), ctypes.byref(ms), ctypes.c_void_p(stream)))
+    with _c_potential(s, potential) as pot:
+        code = s.lib.drift_tune_device(s.handle, method, ctypes.c_void_p(d_state0),
+                                       ctypes.c_void_p(d_times), n, nt, rtol_ln, atol_ln, dt_one,
+                                       ctypes.byref(pot) if pot is not None else None,
+                                       prefix_samples, ctypes.byref(best), ctypes.byref(ms),
+                                       ctypes.c_void_p(stream))
+    check(code)
     return {"schedule": best.schedule, "block_size": best.block_size,
             "ordering": best.ordering, "staging": best.staging, "ms": ms.value}
 
@@ -503,12 +625,32 @@ def ragged_device(jobs: list[DeviceJob], potentials: list, precision: int = PREC
 
 
 def force_device(potential: PotentialSpec | None, d_pos: int, d_acc: int, n: int,
-                 precision: int = PRECISION_FP64, stream: int = 0, device: int = 0) -> None:
+                 precision: int = PRECISION_FP64, stream: int = 0, device: int = 0,
+                 d_t: int = 0) -> None:
+    """drift_force_device: accelerations at n points (d_t: n times, or 0 for
+    t = 0)."""
     s = session(device)
-    pot = potential.to_c() if potential is not None else None
-    check(s.lib.drift_force_device(s.handle, ctypes.byref(pot) if pot is not None else None,
-                                   ctypes.c_void_p(d_pos), ctypes.c_void_p(d_acc), n, precision,
-                                   ctypes.c_void_p(stream)))
+    with _c_potential(s, potential) as pot:
+        code = s.lib.drift_force_device(s.handle, ctypes.byref(pot) if pot is not None else None,
+                                        ctypes.c_void_p(d_pos), ctypes.c_void_p(d_t),
+                                        ctypes.c_void_p(d_acc), n, precision,
+                                        ctypes.c_void_p(stream))
+    check(code)
+
+
+def force_host(potential: PotentialSpec | None, pos, t=None, device: int = 0) -> np.ndarray:
+    """drift_force: FP64 accelerations (n, 3) at host positions pos (n, 3) and
+    times t (n,) or None for t = 0."""
+    s = session(device)
+    p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    n = p.shape[0]
+    tt = None if t is None else np.ascontiguousarray(np.broadcast_to(t, (n,)), dtype=np.float64)
+    acc = np.empty((n, 3))
+    with _c_potential(s, potential) as pot:
+        code = s.lib.drift_force(s.handle, ctypes.byref(pot) if pot is not None else None,
+                                 _ptr(p), _ptr(tt) if tt is not None else None, _ptr(acc), n)
+    check(code)
+    return acc
 
 
 def mw_table_check(potential: PotentialSpec, samples_per_segment: int = 200) -> float:

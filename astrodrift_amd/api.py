@@ -16,7 +16,11 @@ all deliberate:
 * ``Config(..., forces="model")`` integrates each container in the potential
   of its force sources instead of the reference's hard-coded unit galpy Kepler
   force (the default, ``forces="reference"``, keeps the reference behaviour,
-  src/dispatch/gpu.rs:155-156).
+  src/dispatch/gpu.rs:155-156).  A particle group with an attached potential
+  acts as a force source through moving origins (the reference's
+  CustomKepler / CustomPlummer recipes, shared/src/model.rs:180-235, which it
+  never constructs): the group is integrated first and each of its particles
+  becomes a point mass or Plummer sphere following its trajectory.
 """
 
 from __future__ import annotations
@@ -435,28 +439,73 @@ class Config:
 
     # -- execution ----------------------------------------------------------
 
-    def _force_for(self, node: Container) -> _native.PotentialSpec | None:
+    def _sources(self, node: Container) -> list[Container]:
+        return [c for c in self._containers if (c._identity, node._identity) in self._dependencies]
+
+    def _integration_order(self) -> list[Container]:
+        """State-bearing containers, each after the particle groups it feels
+        (forces="model"); first-registration order otherwise."""
+        pending = [c for c in self._containers if c._state is not None]
+        if self.forces == "reference":
+            return pending
+        order: list[Container] = []
+        done: set[int] = set()
+        while pending:
+            ready = next(c for c in pending
+                         if all(s._identity in done for s in self._sources(c)
+                                if s._state is not None))  # add() keeps the graph acyclic
+            order.append(ready)
+            done.add(ready._identity)
+            pending.remove(ready)
+        return order
+
+    def _force_for(self, node: Container, specs: dict | None = None,
+                   outputs: dict | None = None,
+                   times: np.ndarray | None = None) -> _native.PotentialSpec | None:
+        """The force a container feels: None (the reference's unit galpy
+        Kepler) for forces="reference"; otherwise the sum over its force
+        sources, a background's static potential or a particle group's moving
+        origins (one per particle, on quintic Hermite tracks of the group's
+        integrated trajectory)."""
         if self.forces == "reference":
             return None  # unit galpy Kepler, as the reference kernels hard-code
-        sources = [
-            c
-            for c in self._containers
-            if (c._identity, node._identity) in self._dependencies
-        ]
-        statics = []
-        for c in sources:
-            if c._state is not None and c._potential is not None:
-                raise NotImplementedError(
-                    "particle groups as force sources need moving-origin potentials, "
-                    "which are not supported"
-                )
-            if c._potential is not None:
-                statics.append(c._potential)
-        if len(statics) != 1:
+        statics = [c._potential for c in self._sources(node)
+                   if c._state is None and c._potential is not None]
+        groups = [c for c in self._sources(node) if c._state is not None and c._potential is not None]
+        bovy = [p for p in statics if p._name == "bovy"]
+        points = [p for p in statics if p._name != "bovy"]
+        if len(bovy) > 1:
             raise NotImplementedError(
-                "forces='model' needs exactly one background potential per integrated container"
+                "forces='model' supports at most one bovy background per integrated container"
             )
-        return statics[0].spec()
+        if not groups and len(points) + len(bovy) == 1:
+            return statics[0].spec()
+        base = bovy[0].spec() if bovy else _native.PotentialSpec.none()
+        if not groups and not points:
+            return base
+        if times is None:
+            times = self.grid.times()
+        sets = []
+        for g in groups:
+            states = outputs[g._identity]
+            nt, n = states.shape[0], states.shape[1]
+            pos = states[:, :, :3].reshape(-1, 3)
+            t_of = np.repeat(times, n)
+            acc = _native.force_host(specs[g._identity], pos, t_of,
+                                     device=self.device).reshape(nt, n, 3)
+            amp = g._potential._amp
+            b = g._potential._radius if g._potential._name == "plummer" else 0.0
+            try:
+                sets.append(_native.Origins.hermite(states, acc, times, (amp, b)))
+            except _native.DriftNativeError as exc:
+                raise NotImplementedError(
+                    f"particle groups as force sources need a uniform time grid: {exc}"
+                ) from None
+        span = (float(times.min()), float(times.max()))
+        for p in points:
+            b = p._radius if p._name == "plummer" else 0.0
+            sets.append(_native.Origins.fixed((p._amp, b), max(len(times) - 1, 1), *span))
+        return base.with_origins(_native.Origins.concat(sets))
 
     def run(self) -> list:
         """Integrate the registered model.
@@ -495,34 +544,35 @@ class Config:
             self.atol,
         )
         started = time.perf_counter()
-        results: list = []
+        outputs: dict[int, np.ndarray] = {}
+        specs: dict[int, Any] = {}
         stage = 0
         total = 0
-        for c in self._containers:
-            if c._state is None:
-                results.append(None)
-                continue
+        for c in self._integration_order():
             stage += 1
             n = c._state.shape[0]
             total += n
             _log_gpu.debug("stage %d: dispatching %d particles to the GPU", stage, n)
             stage_started = time.perf_counter()
-            potential = self._force_for(c)
-            _log_hip.debug(
-                "launching GALPY %s kernel: %d particles, %d output times", self.method.value, n,
-                self.grid.steps,
-            )
             try:
+                potential = self._force_for(c, specs, outputs, times)
+                _log_hip.debug(
+                    "launching GALPY %s kernel: %d particles, %d output times", self.method.value,
+                    n, self.grid.steps,
+                )
                 out, _stats, _code = _native.launch_host(
                     method, c._state, times, rtol_ln, atol_ln, potential, device=self.device
                 )
             except _native.DriftNativeError as exc:
                 raise IntegrationError(str(exc)) from exc
-            results.append(out)
+            outputs[c._identity] = out
+            specs[c._identity] = potential
             _log_gpu.debug(
                 "stage %d: %d particles finished in %.3fs", stage, n,
                 time.perf_counter() - stage_started,
             )
+        results = [outputs.get(c._identity) if c._state is not None else None
+                   for c in self._containers]
         _log_gpu.info(
             "GPU integration finished: %d stage(s), %d particle trajectories in %.3fs",
             stage,

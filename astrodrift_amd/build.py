@@ -17,7 +17,13 @@ PACKAGE_DIR = Path(__file__).resolve().parent
 REPO_DIR = PACKAGE_DIR.parent
 CSRC = PACKAGE_DIR / "csrc"
 LIB_PATH = PACKAGE_DIR / "libdrift_hip.so"
-SOURCES = [CSRC / "drift_hip.hip", CSRC / "ragged.hip", CSRC / "serving.cpp", CSRC / "spherical_table.cpp"]
+SOURCES = [
+    CSRC / "drift_hip.hip",
+    CSRC / "ragged.hip",
+    CSRC / "serving.cpp",
+    CSRC / "spherical_table.cpp",
+    CSRC / "origins.cpp",
+]
 HEADERS = sorted(CSRC.glob("*.hpp")) + [REPO_DIR / "include" / "drift_hip.h"]
 
 ARCH = "gfx950"

@@ -21,13 +21,28 @@ namespace drift {
 
 constexpr int DIM = 6;
 
-// Potential kinds; values match drift_potential_kind in include/drift_hip.h.
+// Potential kinds; values 0..4 match drift_potential_kind in include/drift_hip.h.
+// POT_MOVING is the kernels' own flag: the static kind plus a set of moving
+// origins (point masses or Plummer spheres on tabulated tracks).
 enum : int32_t {
     POT_GALPY_KEPLER = 0,
     POT_KEPLER = 1,
     POT_PLUMMER = 2,
     POT_MW = 3,
+    POT_NONE = 4,
+    POT_MOVING = 8,
 };
+
+// Static kind of a kernel kind (without the moving-origin flag).
+constexpr int32_t static_kind(int32_t kind) { return kind & ~POT_MOVING; }
+// Whether a kernel kind stages the Milky-Way table in LDS.
+constexpr bool kind_has_table(int32_t kind) { return static_kind(kind) == POT_MW; }
+
+// Moving-origin tracks: for origin i and time segment s, ORIGIN_COEFS doubles
+// at ((i * n_segments) + s) * ORIGIN_COEFS hold the x, y and z quintics in
+// the segment's local time tau in [0, 1), each as (tau^5, tau^4, ..., tau^0)
+// coefficients.
+constexpr int ORIGIN_COEFS = 18;
 
 // Spherical-part table of the Milky-Way model (bulge + NFW halo):
 //   a_sph(r) = -S(r^2) * r_vec,
@@ -70,6 +85,12 @@ struct PotParams {
     double b2;        // Plummer b^2
     double disk_amp, disk_a, disk_b2;
     const double* far_table;  // MW: FAR_DOUBLES coefficients in device memory
+    // moving origins (kind | POT_MOVING), device memory
+    const double* tracks;         // n_origins x n_segments x ORIGIN_COEFS
+    const double* origin_params;  // n_origins x (amp, b^2)
+    int32_t n_origins, n_segments;
+    double t_start;  // start of segment 0
+    double inv_dt;   // segments per unit time
 };
 
 DRIFT_HD DRIFT_FI double rsqrt_d(double x) {

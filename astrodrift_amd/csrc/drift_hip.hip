@@ -102,10 +102,31 @@ struct ForceFor<POT_MW, FT> {
     }
 };
 
+template <class FT>
+struct ForceFor<POT_NONE, FT> {
+    using type = NoForce;
+    __device__ static type make(const PotParams&, const FT*) { return type{}; }
+};
+
+// The force of a kernel kind: the static force, or with POT_MOVING set the
+// static force plus the moving origins.
+template <int KIND, class FT, bool MOVING = (KIND & POT_MOVING) != 0>
+struct KindForce : ForceFor<KIND, FT> {};
+
+template <int KIND, class FT>
+struct KindForce<KIND, FT, true> {
+    using Static = ForceFor<static_kind(KIND), FT>;
+    using type = MovingForce<typename Static::type>;
+    __device__ static type make(const PotParams& p, const FT* table) {
+        return type{Static::make(p, table), p.tracks, p.origin_params, p.n_origins, p.n_segments,
+                    p.t_start, p.inv_dt};
+    }
+};
+
 // Stages the MW coefficient table into LDS (no-op for the other potentials).
 template <int KIND, class FT>
 __device__ __forceinline__ const FT* stage_table(const void* gtable, FT* lds) {
-    if constexpr (KIND == POT_MW) {
+    if constexpr (kind_has_table(KIND)) {
         const FT* g = static_cast<const FT*>(gtable);
         constexpr int words = sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS;
         for (int k = threadIdx.x; k < words; k += blockDim.x) lds[k] = g[k];
@@ -131,12 +152,12 @@ constexpr int kDefaultBlock = 256;
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_plain_kernel(PlainArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
+    __shared__ FT lds[kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= a.n) return;
     const int64_t i = a.order ? (int64_t)a.order[slot] : slot;
-    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    const auto f = KindForce<KIND, FT>::make(a.pot, table);
     const double t0 = a.times[0];
     const Dop853Job J = make_dop853_job(a.times, a.nt, a.rtol, a.atol, a.max_attempts, t0,
                                         a.times[a.nt - 1]);
@@ -159,12 +180,12 @@ __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_plain_ke
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) dopr54_plain_kernel(PlainArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
+    __shared__ FT lds[kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= a.n) return;
     const int64_t i = a.order ? (int64_t)a.order[slot] : slot;
-    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    const auto f = KindForce<KIND, FT>::make(a.pot, table);
     Dopr54Job J;
     J.times = a.times;
     J.nt = a.nt;
@@ -216,9 +237,9 @@ __device__ __forceinline__ bool claim_orbit(bool need, unsigned long long* queue
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_persistent_kernel(PlainArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
+    __shared__ FT lds[kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
-    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    const auto f = KindForce<KIND, FT>::make(a.pot, table);
     const double t0 = a.times[0];
     const Dop853Job J = make_dop853_job(a.times, a.nt, a.rtol, a.atol, a.max_attempts, t0,
                                         a.times[a.nt - 1]);
@@ -251,9 +272,9 @@ __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_persiste
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) dopr54_persistent_kernel(PlainArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
+    __shared__ FT lds[kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
-    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    const auto f = KindForce<KIND, FT>::make(a.pot, table);
     Dopr54Job J;
     J.times = a.times;
     J.nt = a.nt;
@@ -359,17 +380,17 @@ __global__ void order_reset_kernel(uint32_t* bins) {
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) order_key_kernel(PlainArgs a, uint16_t* keys,
                                                                    uint32_t* bins) {
-    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
+    __shared__ FT lds[kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     __shared__ uint32_t local[kOrderBins];
     for (int k = threadIdx.x; k < kOrderBins; k += blockDim.x) local[k] = 0u;
     const FT* table = stage_table<KIND, FT>(a.table, lds);  // ends with a barrier
-    if constexpr (KIND != POT_MW) __syncthreads();
+    if constexpr (!kind_has_table(KIND)) __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.n) {
-        const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+        const auto f = KindForce<KIND, FT>::make(a.pot, table);
         double y[DIM], d[DIM];
         load_state(a.state0, i, y);
-        f(y, d);
+        f(a.times[0], y, d);
         const uint32_t b = cost_bin(y, d);
         keys[i] = (uint16_t)b;
         atomicAdd(&local[b], 1u);
@@ -433,6 +454,7 @@ __global__ void __launch_bounds__(kScatterThreads) order_scatter_kernel(const ui
 
 struct ForceArgs {
     const double* pos;
+    const double* t;  // per-point times, or NULL for t = 0
     double* acc;
     int64_t n;
     PotParams pot;
@@ -441,14 +463,14 @@ struct ForceArgs {
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) force_kernel(ForceArgs a) {
-    __shared__ FT lds[KIND == POT_MW ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
+    __shared__ FT lds[kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    const auto f = ForceFor<KIND, FT>::make(a.pot, table);
+    const auto f = KindForce<KIND, FT>::make(a.pot, table);
     double q[DIM] = {a.pos[3 * i], a.pos[3 * i + 1], a.pos[3 * i + 2], 0.0, 0.0, 0.0};
     double d[DIM];
-    f(q, d);
+    f(a.t ? a.t[i] : 0.0, q, d);
     a.acc[3 * i] = d[3];
     a.acc[3 * i + 1] = d[4];
     a.acc[3 * i + 2] = d[5];
@@ -487,16 +509,26 @@ namespace {
 
 bool finite(double x) { return std::isfinite(x); }
 
-bool same_potential(const drift_potential& a, const drift_potential& b) {
+// Whether two descriptors have the same static force (the moving-origin set
+// does not enter the static force's tables).
+bool same_potential(drift_potential a, drift_potential b) {
+    a.origins = 0;
+    b.origins = 0;
     return std::memcmp(&a, &b, sizeof(drift_potential)) == 0;
 }
 
 }  // namespace
 
 int validate_potential(const drift_potential& p) {
-    if (p.reserved != 0) return fail(DRIFT_ERR_INVALID_ARGUMENT, "drift_potential.reserved must be 0");
+    if (p.origins < 0)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "drift_potential.origins must be 0 or a set id");
     switch (p.kind) {
         case POT_GALPY_KEPLER:
+            if (p.origins != 0)
+                return fail(DRIFT_ERR_UNSUPPORTED,
+                            "moving origins cannot be added to the galpy Kepler force");
+            return DRIFT_OK;
+        case POT_NONE:
             return DRIFT_OK;
         case POT_KEPLER:
             if (!finite(p.amp)) return fail(DRIFT_ERR_INVALID_ARGUMENT, "Kepler amp must be finite");
@@ -537,15 +569,46 @@ int resolve_potential(drift_session* s, const drift_potential* potential, PotPar
     *table32 = nullptr;
     params = PotParams{};
     params.kind = p.kind;
+    // moving origins (a static NONE kind runs as an empty moving set)
+    if (p.origins != 0 || p.kind == POT_NONE) {
+        params.kind |= POT_MOVING;
+        params.n_segments = 1;
+        if (p.origins != 0) {
+            const OriginSet* set = nullptr;
+            for (const auto& o : s->origin_sets)
+                if (o.id == p.origins) set = &o;
+            if (!set)
+                return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                            "drift_potential.origins " + std::to_string(p.origins) +
+                                " is not a moving-origin set of this session");
+            params.tracks = set->d_tracks;
+            params.origin_params = set->d_params;
+            params.n_origins = set->n_origins;
+            params.n_segments = set->n_segments;
+            params.t_start = set->t_start;
+            params.inv_dt = (double)set->n_segments / (set->t_end - set->t_start);
+        }
+    }
     if (p.kind == POT_KEPLER) {
         params.amp = p.amp;
     } else if (p.kind == POT_PLUMMER) {
         params.amp = p.amp;
         params.b2 = p.b * p.b;
     } else if (p.kind == POT_MW) {
+        const PotParams moving = params;
+        auto with_origins = [&](const PotParams& mw) {
+            params = mw;
+            params.kind = moving.kind;
+            params.tracks = moving.tracks;
+            params.origin_params = moving.origin_params;
+            params.n_origins = moving.n_origins;
+            params.n_segments = moving.n_segments;
+            params.t_start = moving.t_start;
+            params.inv_dt = moving.inv_dt;
+        };
         for (auto& e : s->tables) {
             if (same_potential(e.key, p)) {
-                params = e.params;
+                with_origins(e.params);
                 *table64 = e.d_table64;
                 *table32 = e.d_table32;
                 return DRIFT_OK;
@@ -553,6 +616,7 @@ int resolve_potential(drift_session* s, const drift_potential* potential, PotPar
         }
         TableEntry e;
         e.key = p;
+        e.key.origins = 0;
         build_mw_params(p, e.params);
         std::vector<double> h64(TABLE_DOUBLES), far(FAR_DOUBLES);
         build_mw_table(p, h64.data());
@@ -580,7 +644,7 @@ int resolve_potential(drift_session* s, const drift_potential* potential, PotPar
         }
         e.params.far_table = e.d_far;
         s->tables.push_back(e);
-        params = e.params;
+        with_origins(e.params);
         *table64 = e.d_table64;
         *table32 = e.d_table32;
     }
@@ -604,12 +668,19 @@ PlainKernel pick_plain(int method, int precision, int schedule) {
                                               : pick_schedule<KIND, double>(method, schedule);
 }
 
+// Moving-origin kinds exist in FP64 only (launch_integrator refuses MIXED).
 PlainKernel kernel_for(int method, int kind, int precision, int schedule) {
     switch (kind) {
         case POT_GALPY_KEPLER: return pick_plain<POT_GALPY_KEPLER>(method, precision, schedule);
         case POT_KEPLER: return pick_plain<POT_KEPLER>(method, precision, schedule);
         case POT_PLUMMER: return pick_plain<POT_PLUMMER>(method, precision, schedule);
-        default: return pick_plain<POT_MW>(method, precision, schedule);
+        case POT_MW: return pick_plain<POT_MW>(method, precision, schedule);
+        case POT_MOVING | POT_KEPLER:
+            return pick_schedule<POT_MOVING | POT_KEPLER, double>(method, schedule);
+        case POT_MOVING | POT_PLUMMER:
+            return pick_schedule<POT_MOVING | POT_PLUMMER, double>(method, schedule);
+        case POT_MOVING | POT_MW: return pick_schedule<POT_MOVING | POT_MW, double>(method, schedule);
+        default: return pick_schedule<POT_MOVING | POT_NONE, double>(method, schedule);
     }
 }
 
@@ -626,7 +697,11 @@ KeyKernel key_kernel_for(int kind, int precision) {
         case POT_GALPY_KEPLER: return pick_key<POT_GALPY_KEPLER>(precision);
         case POT_KEPLER: return pick_key<POT_KEPLER>(precision);
         case POT_PLUMMER: return pick_key<POT_PLUMMER>(precision);
-        default: return pick_key<POT_MW>(precision);
+        case POT_MW: return pick_key<POT_MW>(precision);
+        case POT_MOVING | POT_KEPLER: return order_key_kernel<POT_MOVING | POT_KEPLER, double>;
+        case POT_MOVING | POT_PLUMMER: return order_key_kernel<POT_MOVING | POT_PLUMMER, double>;
+        case POT_MOVING | POT_MW: return order_key_kernel<POT_MOVING | POT_MW, double>;
+        default: return order_key_kernel<POT_MOVING | POT_NONE, double>;
     }
 }
 
@@ -769,6 +844,9 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     const void *t64 = nullptr, *t32 = nullptr;
     int rc = resolve_potential(s, potential, a.pot, &t64, &t32);
     if (rc != DRIFT_OK) return rc;
+    if ((a.pot.kind & POT_MOVING) && opt.precision == DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "moving origins and the NONE potential need FP64 precision");
     a.table = opt.precision == DRIFT_PRECISION_MIXED ? t32 : t64;
     if (opt.schedule == DRIFT_SCHEDULE_TUNED) {
         const TunedEntry* e = find_tuned(s, method, a.pot.kind, opt.precision, n);
@@ -947,9 +1025,13 @@ int drift_session_destroy(drift_session* s) {
         (void)hipFree(t.d_table32);
         (void)hipFree(t.d_far);
     }
+    for (auto& o : s->origin_sets) {
+        (void)hipFree(o.d_tracks);
+        (void)hipFree(o.d_params);
+    }
     for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats, &s->order_keys,
                             &s->order_slots, &s->order_bins, &s->tune_out, &s->tune_stats,
-                            &s->staging})
+                            &s->staging, &s->force_io})
         if (b->ptr) (void)hipFree(b->ptr);
     if (s->order_done) (void)hipEventDestroy(s->order_done);
     if (s->staging_done) (void)hipEventDestroy(s->staging_done);
@@ -1121,9 +1203,71 @@ int drift_tune_device(drift_session* s, int32_t method, const double* d_state0,
     return DRIFT_OK;
 }
 
+int drift_origins_create(drift_session* session, const double* tracks, const double* params,
+                         int32_t n_origins, int32_t n_segments, double t_start, double t_end,
+                         int32_t* origins_id) {
+    if (!session || !tracks || !params || !origins_id)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n_origins < 1 || n_segments < 1)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "n_origins and n_segments must be >= 1");
+    if (!std::isfinite(t_start) || !std::isfinite(t_end) || !(t_end > t_start) || !std::isfinite(t_end - t_start))
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "the track span must be finite with t_end > t_start");
+    const size_t ncoef = (size_t)n_origins * (size_t)n_segments * ORIGIN_COEFS;
+    for (size_t k = 0; k < ncoef; ++k)
+        if (!std::isfinite(tracks[k]))
+            return fail(DRIFT_ERR_INVALID_ARGUMENT, "track coefficients must be finite");
+    std::vector<double> pb(2 * (size_t)n_origins);
+    for (int32_t i = 0; i < n_origins; ++i) {
+        const double amp = params[2 * i], b = params[2 * i + 1];
+        if (!std::isfinite(amp) || !std::isfinite(b) || b < 0.0)
+            return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                        "origin amp must be finite and b finite and >= 0");
+        pb[2 * i] = amp;
+        pb[2 * i + 1] = b * b;
+    }
+    DRIFT_HIP_TRY(hipSetDevice(session->device));
+    OriginSet o;
+    o.n_origins = n_origins;
+    o.n_segments = n_segments;
+    o.t_start = t_start;
+    o.t_end = t_end;
+    hipError_t err = hipMalloc(&o.d_tracks, ncoef * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc(&o.d_params, pb.size() * sizeof(double));
+    if (err == hipSuccess)
+        err = hipMemcpy(o.d_tracks, tracks, ncoef * sizeof(double), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(o.d_params, pb.data(), pb.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        if (o.d_tracks) (void)hipFree(o.d_tracks);
+        if (o.d_params) (void)hipFree(o.d_params);
+        set_last_error(std::string("HIP error: ") + hipGetErrorString(err) +
+                       " uploading moving-origin tracks");
+        return err == hipErrorOutOfMemory ? DRIFT_ERR_OUT_OF_MEMORY : DRIFT_ERR_DEVICE;
+    }
+    o.id = session->next_origin_id++;
+    session->origin_sets.push_back(o);
+    *origins_id = o.id;
+    return DRIFT_OK;
+}
+
+int drift_origins_destroy(drift_session* session, int32_t origins_id) {
+    if (!session) return fail(DRIFT_ERR_INVALID_ARGUMENT, "session is NULL");
+    for (size_t k = 0; k < session->origin_sets.size(); ++k) {
+        OriginSet& o = session->origin_sets[k];
+        if (o.id != origins_id) continue;
+        DRIFT_HIP_TRY(hipSetDevice(session->device));
+        DRIFT_HIP_TRY(hipFree(o.d_tracks));
+        DRIFT_HIP_TRY(hipFree(o.d_params));
+        session->origin_sets.erase(session->origin_sets.begin() + (ptrdiff_t)k);
+        return DRIFT_OK;
+    }
+    return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                "no moving-origin set " + std::to_string(origins_id) + " in this session");
+}
+
 int drift_force_device(drift_session* session, const drift_potential* potential,
-                       const double* d_pos, double* d_acc, size_t n, int32_t precision,
-                       void* stream) {
+                       const double* d_pos, const double* d_t, double* d_acc, size_t n,
+                       int32_t precision, void* stream) {
     if (!session) return fail(DRIFT_ERR_INVALID_ARGUMENT, "session is NULL");
     if (!d_pos || !d_acc || n < 1) return fail(DRIFT_ERR_INVALID_ARGUMENT, "bad force arguments");
     if (precision != DRIFT_PRECISION_FP64 && precision != DRIFT_PRECISION_MIXED)
@@ -1131,12 +1275,16 @@ int drift_force_device(drift_session* session, const drift_potential* potential,
     DRIFT_HIP_TRY(hipSetDevice(session->device));
     ForceArgs a{};
     a.pos = d_pos;
+    a.t = d_t;
     a.acc = d_acc;
     a.n = (int64_t)n;
     const void *t64 = nullptr, *t32 = nullptr;
     int rc = resolve_potential(session, potential, a.pot, &t64, &t32);
     if (rc != DRIFT_OK) return rc;
     const bool mixed = precision == DRIFT_PRECISION_MIXED;
+    if ((a.pot.kind & POT_MOVING) && mixed)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "moving origins and the NONE potential need FP64 precision");
     a.table = mixed ? t32 : t64;
     using FK = void (*)(ForceArgs);
     FK k;
@@ -1150,14 +1298,39 @@ int drift_force_device(drift_session* session, const drift_potential* potential,
         case POT_PLUMMER:
             k = mixed ? force_kernel<POT_PLUMMER, float> : force_kernel<POT_PLUMMER, double>;
             break;
-        default:
+        case POT_MW:
             k = mixed ? force_kernel<POT_MW, float> : force_kernel<POT_MW, double>;
             break;
+        case POT_MOVING | POT_KEPLER: k = force_kernel<POT_MOVING | POT_KEPLER, double>; break;
+        case POT_MOVING | POT_PLUMMER: k = force_kernel<POT_MOVING | POT_PLUMMER, double>; break;
+        case POT_MOVING | POT_MW: k = force_kernel<POT_MOVING | POT_MW, double>; break;
+        default: k = force_kernel<POT_MOVING | POT_NONE, double>; break;
     }
     const int64_t blocks = ((int64_t)n + kDefaultBlock - 1) / kDefaultBlock;
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kDefaultBlock), 0,
                        stream ? (hipStream_t)stream : session->stream, a);
     DRIFT_HIP_TRY(hipGetLastError());
+    return DRIFT_OK;
+}
+
+int drift_force(drift_session* session, const drift_potential* potential, const double* pos,
+                const double* t, double* acc, size_t n) {
+    if (!session) return fail(DRIFT_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (!pos || !acc || n < 1) return fail(DRIFT_ERR_INVALID_ARGUMENT, "bad force arguments");
+    DRIFT_HIP_TRY(hipSetDevice(session->device));
+    const size_t vec = n * 3 * sizeof(double), times = t ? n * sizeof(double) : 0;
+    int rc = ensure_buffer(session->force_io, 2 * vec + times);
+    if (rc != DRIFT_OK) return rc;
+    double* d_pos = static_cast<double*>(session->force_io.ptr);
+    double* d_acc = d_pos + 3 * n;
+    double* d_t = t ? d_acc + 3 * n : nullptr;
+    hipStream_t st = session->stream;
+    DRIFT_HIP_TRY(hipMemcpyAsync(d_pos, pos, vec, hipMemcpyHostToDevice, st));
+    if (t) DRIFT_HIP_TRY(hipMemcpyAsync(d_t, t, times, hipMemcpyHostToDevice, st));
+    rc = drift_force_device(session, potential, d_pos, d_t, d_acc, n, DRIFT_PRECISION_FP64, st);
+    if (rc != DRIFT_OK) return rc;
+    DRIFT_HIP_TRY(hipMemcpyAsync(acc, d_acc, vec, hipMemcpyDeviceToHost, st));
+    DRIFT_HIP_TRY(hipStreamSynchronize(st));
     return DRIFT_OK;
 }
 

@@ -25,6 +25,15 @@ struct DeviceBuffer {
     size_t bytes = 0;
 };
 
+// A moving-origin set uploaded by drift_origins_create.
+struct OriginSet {
+    int32_t id = 0;
+    double* d_tracks = nullptr;  // n_origins x n_segments x ORIGIN_COEFS
+    double* d_params = nullptr;  // n_origins x (amp, b^2)
+    int32_t n_origins = 0, n_segments = 0;
+    double t_start = 0.0, t_end = 0.0;
+};
+
 // A launch configuration chosen by drift_tune_device.
 struct TunedEntry {
     int method, kind, precision, log2n;
@@ -60,6 +69,11 @@ struct drift_session {
     // staging_done marks the end of the last launch that used it
     drift::DeviceBuffer staging;
     hipEvent_t staging_done = nullptr;
+    // drift_force's device copies of its host buffers
+    drift::DeviceBuffer force_io;
+    // moving-origin sets (drift_origins_create)
+    std::vector<drift::OriginSet> origin_sets;
+    int32_t next_origin_id = 1;
 };
 
 #define DRIFT_HIP_TRY(expr)                                                              \
@@ -96,6 +110,10 @@ void build_mw_far_table(const drift_potential& p, double* coeffs);
 double mw_far_table_max_rel_error(const drift_potential& p, const double* coeffs,
                                   int samples_per_seg);
 double mw_spherical_exact(const drift_potential& p, double r2);
+
+// Quintic Hermite tracks (origins.cpp).
+int origin_tracks_hermite(const double* states, const double* accels, const double* times,
+                          size_t nt, size_t n, double* tracks, double* t_start, double* t_end);
 
 // Thread-local error message (drift_last_error).
 void set_last_error(const std::string& msg);

@@ -1,8 +1,10 @@
 // forces.hpp -- right-hand sides a = (v, F(x)) for the gfx950 integrators.
 //
-// Each force is a small value type whose operator() fills a[6] from q[6]; the
-// integrator cores are templated on it so the force is inlined into the step
-// with no indirect calls.  FT is the arithmetic type of the force itself:
+// Each force is a small value type whose operator() fills a[6] from the time t
+// and q[6], in the shape of the reference's right-hand side FuncPtr(t, q, a)
+// (src/integrators/galpy/dop853.rs:23-31); the integrator cores are templated
+// on it so the force is inlined into the step with no indirect calls, and the
+// stage times are dead code for the static forces, which ignore t.  FT is the arithmetic type of the force itself:
 // double for the FP64 path, float for the mixed-precision path (positions are
 // rounded to float, accelerations widened back to double).
 #pragma once
@@ -13,7 +15,7 @@ namespace drift {
 
 template <class FT>
 struct GalpyKeplerForce {
-    DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
+    DRIFT_HD DRIFT_FI void operator()(double, const double* q, double* a) const {
         FT ax, ay, az;
         galpy_kepler_acc<FT>(FT(q[0]), FT(q[1]), FT(q[2]), ax, ay, az);
         a[0] = q[3];
@@ -28,7 +30,7 @@ struct GalpyKeplerForce {
 template <class FT>
 struct KeplerForce {
     FT amp;
-    DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
+    DRIFT_HD DRIFT_FI void operator()(double, const double* q, double* a) const {
         FT ax, ay, az;
         kepler_acc<FT>(amp, FT(q[0]), FT(q[1]), FT(q[2]), ax, ay, az);
         a[0] = q[3];
@@ -43,7 +45,7 @@ struct KeplerForce {
 template <class FT>
 struct PlummerForce {
     FT amp, b2;
-    DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
+    DRIFT_HD DRIFT_FI void operator()(double, const double* q, double* a) const {
         FT ax, ay, az;
         plummer_acc<FT>(amp, b2, FT(q[0]), FT(q[1]), FT(q[2]), ax, ay, az);
         a[0] = q[3];
@@ -64,7 +66,7 @@ struct MwForce {
     const PotParams* p;  // far-table pointer (outside the LDS table's range)
     const FT* table;
     FT disk_amp, disk_a, disk_b2;
-    DRIFT_HD DRIFT_FI void operator()(const double* q, double* a) const {
+    DRIFT_HD DRIFT_FI void operator()(double, const double* q, double* a) const {
         const FT x = FT(q[0]), y = FT(q[1]), z = FT(q[2]);
         // Miyamoto-Nagai disk (shared/src/potential.rs:138-151), in the
         // factored form a_R = f R, a_z = f z (a + zb) / zb with
@@ -102,6 +104,71 @@ struct MwForce {
         a[3] = double(g * x);
         a[4] = double(g * y);
         a[5] = double((f * (azb * izb) - S) * z);
+    }
+};
+
+// No static force: the background of a potential made of moving origins only
+// (or free motion).
+struct NoForce {
+    DRIFT_HD DRIFT_FI void operator()(double, const double* q, double* a) const {
+        a[0] = q[3];
+        a[1] = q[4];
+        a[2] = q[5];
+        a[3] = 0.0;
+        a[4] = 0.0;
+        a[5] = 0.0;
+    }
+};
+
+// Static force plus moving origins, after CustomOrigin::force
+// (shared/src/potential.rs:334-356): the force of a point mass (b = 0,
+// KeplerPotential) or Plummer sphere centred on each origin's position o_i(t),
+// a = sum_i F_i(x - o_i(t)) summed in origin order, then added to the static
+// force.  The
+// origins follow quintic tracks over n_segments equal time segments from
+// t_start (CustomOrigin::origins, :290-331): segment s = floor(u) of
+// u = (t - t_start) inv_dt, clamped to [0, n_segments - 1] so a time outside
+// the tabulated span extrapolates its end segment, at local time tau = u - s.
+// FP64 only.
+template <class Base>
+struct MovingForce {
+    Base base;
+    const double* tracks;
+    const double* origin_params;
+    int32_t n_origins, n_segments;
+    double t_start, inv_dt;
+    DRIFT_HD DRIFT_FI void operator()(double t, const double* q, double* a) const {
+        base(t, q, a);
+        const double u = (t - t_start) * inv_dt;
+        double sf = std::floor(u);
+        if (!(sf >= 0.0)) sf = 0.0;  // also a NaN time: its tau stays NaN
+        const double last = (double)(n_segments - 1);
+        if (sf > last) sf = last;
+        const double tau = u - sf;
+        const int64_t seg = (int64_t)sf;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int32_t i = 0; i < n_origins; ++i) {
+            // rows are 144 bytes from a 256-byte aligned base: 16-byte loads
+            const double* c = static_cast<const double*>(__builtin_assume_aligned(
+                tracks + ((int64_t)i * n_segments + seg) * ORIGIN_COEFS, 16));
+            double o[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double* p = c + 6 * k;  // (tau^5 .. tau^0)
+                o[k] = fma(fma(fma(fma(fma(p[0], tau, p[1]), tau, p[2]), tau, p[3]), tau, p[4]),
+                           tau, p[5]);
+            }
+            const double* pb =
+                static_cast<const double*>(__builtin_assume_aligned(origin_params + 2 * i, 16));
+            double fx, fy, fz;
+            plummer_acc<double>(pb[0], pb[1], q[0] - o[0], q[1] - o[1], q[2] - o[2], fx, fy, fz);
+            ax += fx;
+            ay += fy;
+            az += fz;
+        }
+        a[3] += ax;
+        a[4] += ay;
+        a[5] += az;
     }
 };
 

@@ -218,7 +218,7 @@ DRIFT_HD DRIFT_FI void dop853_init(Dop853Lane& L, const double* y0, double t0, c
     for (int i = 0; i < DIM; i++) L.y[i] = y0[i];
     sink.emit(0, L.y);
     double k1[DIM];
-    f(L.y, k1);
+    f(t0, L.y, k1);
 #pragma unroll
     for (int c = 0; c < 3; c++) L.F1[c] = k1[3 + c];
     double dnf = 0.0, dny = 0.0;
@@ -234,7 +234,7 @@ DRIFT_HD DRIFT_FI void dop853_init(Dop853Lane& L, const double* y0, double t0, c
     double k3[DIM], k2[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; i++) k3[i] = L.y[i] + h * k1[i];
-    f(k3, k2);
+    f(t0 + h, k3, k2);
     double der2 = 0.0;
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
@@ -286,11 +286,13 @@ DRIFT_HD DRIFT_FI void rkn853_load(Dop853Coef C, double* L) {
 // X_I = x0 + h c_I v0 + h^2 sum_m (A^2)_Im F_m (the stage velocity, which the
 // force does not read, is never formed) and its force F_I.  L holds stage I's
 // coefficients; stage NEXT's are fetched into Lnext before the force call,
-// so their scalar loads overlap the force evaluation.
+// so their scalar loads overlap the force evaluation.  The force is taken at
+// t_step + c_I h (dop853.rs:432-578 and :689-727, t_step being the step's
+// start).
 template <int I, int NEXT, class F>
-DRIFT_HD DRIFT_FI void rkn853_stage(const Dop853Job& J, const double* y0, double h, double hh,
-                                    double (&Fs)[17][3], const double* L, double* Lnext,
-                                    const F& f) {
+DRIFT_HD DRIFT_FI void rkn853_stage(const Dop853Job& J, double t_step, const double* y0, double h,
+                                    double hh, double (&Fs)[17][3], const double* L,
+                                    double* Lnext, const F& f) {
     const double hc = h * L[0];
     double q[DIM], a[DIM];
 #pragma unroll
@@ -302,7 +304,7 @@ DRIFT_HD DRIFT_FI void rkn853_stage(const Dop853Job& J, const double* y0, double
         q[3 + c] = 0.0;
     }
     if constexpr (NEXT > 0) rkn853_load<NEXT>(dop853_coef(J.coef, q[2]), Lnext);
-    f(q, a);
+    f(t_step + hc, q, a);
 #pragma unroll
     for (int c = 0; c < 3; c++) Fs[I][c] = a[3 + c];
 }
@@ -323,6 +325,7 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
     const double h = J.pos_neg * tmax(std::fabs(L.h), 1e3 * uround);
     const double hh = h * h;
     const double* y0 = L.y;
+    const double t_step = L.t_cur;  // start of this step
     double Fs[17][3];
 #pragma unroll
     for (int c = 0; c < 3; c++) Fs[1][c] = L.F1[c];
@@ -330,20 +333,19 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
     // stage coefficients, double-buffered: each stage fetches the next one's
     double Lc0[12], Lc1[12];
     rkn853_load<2>(dop853_coef(J.coef, h), Lc0);
-    rkn853_stage<2, 3>(J, y0, h, hh, Fs, Lc0, Lc1, f);
-    rkn853_stage<3, 4>(J, y0, h, hh, Fs, Lc1, Lc0, f);
-    rkn853_stage<4, 5>(J, y0, h, hh, Fs, Lc0, Lc1, f);
-    rkn853_stage<5, 6>(J, y0, h, hh, Fs, Lc1, Lc0, f);
-    rkn853_stage<6, 7>(J, y0, h, hh, Fs, Lc0, Lc1, f);
-    rkn853_stage<7, 8>(J, y0, h, hh, Fs, Lc1, Lc0, f);
-    rkn853_stage<8, 9>(J, y0, h, hh, Fs, Lc0, Lc1, f);
-    rkn853_stage<9, 10>(J, y0, h, hh, Fs, Lc1, Lc0, f);
-    rkn853_stage<10, 11>(J, y0, h, hh, Fs, Lc0, Lc1, f);
-    rkn853_stage<11, 12>(J, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<2, 3>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<3, 4>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<4, 5>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<5, 6>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<6, 7>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<7, 8>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<8, 9>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<9, 10>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<10, 11>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<11, 12>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
     const double t_old_older = L.t_old;
-    const double t_step = L.t_cur;  // start of this step
     const double t_new = L.t_cur + h;
-    rkn853_stage<12, 0>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<12, 0>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
 
     // k4 = sum_j b_j k_j and the 8th-order solution k5 = y0 + h k4
     // (dop853.rs:585-595); the position half of k4 is v0 + h bpos(F)
@@ -408,7 +410,7 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
         rkn853_load<14>(dop853_coef(J.coef, k5[2]), Lc0);
         {
             double a[DIM];
-            f(k5, a);
+            f(t_new, k5, a);
 #pragma unroll
             for (int c = 0; c < 3; c++) Fs[13][c] = a[3 + c];
         }
@@ -422,9 +424,9 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
             L.dense++;
             double r1[DIM], r2[DIM], r3[DIM], r4[DIM], r5[DIM], r6[DIM], r7[DIM], r8[DIM];
             // the three extra stages (dop853.rs:703-742)
-            rkn853_stage<14, 15>(J, y0, h, hh, Fs, Lc0, Lc1, f);
-            rkn853_stage<15, 16>(J, y0, h, hh, Fs, Lc1, Lc0, f);
-            rkn853_stage<16, 0>(J, y0, h, hh, Fs, Lc0, Lc1, f);
+            rkn853_stage<14, 15>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
+            rkn853_stage<15, 16>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
+            rkn853_stage<16, 0>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
             // rcont5..8 after the stages, one coefficient group at a time (the
             // groups' scalar loads then fit the scalar registers); the forces
             // die as the sums consume them
@@ -580,24 +582,24 @@ inline void dopr54_bounds(double* bound2) {
 
 // rk4_onestep, dopr54.rs:135-208.
 template <class F>
-DRIFT_HD DRIFT_FI void rk4_onestep(const double* yn, double* yn1, double dt, double* ynk, double* a,
-                                   const F& f) {
-    f(yn, a);
+DRIFT_HD DRIFT_FI void rk4_onestep(const double* yn, double* yn1, double tn, double dt, double* ynk,
+                                   double* a, const F& f) {
+    f(tn, yn, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 6.0;
 #pragma unroll
     for (int i = 0; i < DIM; i++) ynk[i] = yn[i] + dt * a[i] / 2.0;
-    f(ynk, a);
+    f(tn + dt / 2.0, ynk, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 3.0;
 #pragma unroll
     for (int i = 0; i < DIM; i++) ynk[i] = yn[i] + dt * a[i] / 2.0;
-    f(ynk, a);
+    f(tn + dt / 2.0, ynk, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 3.0;
 #pragma unroll
     for (int i = 0; i < DIM; i++) ynk[i] = yn[i] + dt * a[i];
-    f(ynk, a);
+    f(tn + dt, ynk, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 6.0;
 }
@@ -611,19 +613,19 @@ DRIFT_HD DRIFT_FI double max_abs6(const double* y) {
 
 // rk4_estimate_step, dopr54.rs:210-368, with the scale e^s = atol + rtol max|y|.
 template <class F>
-DRIFT_HD DRIFT_FI double rk4_estimate_step(const double* yo, double dt, const Dopr54Job& J,
-                                           const F& f) {
+DRIFT_HD DRIFT_FI double rk4_estimate_step(const double* yo, double dt, double to,
+                                           const Dopr54Job& J, const F& f) {
     const double init_dt = dt;
     const double sc = J.atol + J.rtol * max_abs6(yo);
     double yn[DIM], y1[DIM], y21[DIM], y2[DIM], ynk[DIM], a[DIM];
     for (int iter = 0; iter < 64; ++iter) {
 #pragma unroll
         for (int i = 0; i < DIM; i++) yn[i] = y1[i] = y21[i] = yo[i];
-        rk4_onestep(yn, y1, dt, ynk, a, f);
-        rk4_onestep(yn, y21, dt / 2.0, ynk, a, f);
+        rk4_onestep(yn, y1, to, dt, ynk, a, f);
+        rk4_onestep(yn, y21, to, dt / 2.0, ynk, a, f);
 #pragma unroll
         for (int i = 0; i < DIM; i++) y2[i] = y21[i];
-        rk4_onestep(y21, y2, dt / 2.0, ynk, a, f);
+        rk4_onestep(y21, y2, to + dt / 2.0, dt / 2.0, ynk, a, f);
         double e = 0.0;
 #pragma unroll
         for (int i = 0; i < DIM; i++) {
@@ -666,9 +668,9 @@ DRIFT_HD DRIFT_FI void dopr54_init(Dopr54Lane& L, const double* y0, double t0, c
     for (int i = 0; i < DIM; i++) L.y[i] = y0[i];
     sink.emit(0, L.y);
     double dt_one = J.dt_one_init;
-    if (dt_one == dp54::initial_step_sentinel) dt_one = rk4_estimate_step(L.y, J.dt, J, f);
+    if (dt_one == dp54::initial_step_sentinel) dt_one = rk4_estimate_step(L.y, J.dt, t0, J, f);
     L.to = t0;
-    f(L.y, L.a1);
+    f(t0, L.y, L.a1);
     L.dt_one = dt_one;
     L.init_to = t0;
     L.init_dt_one = dt_one;
@@ -696,6 +698,7 @@ DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F
     if (dt < 0.0 && dt_one < (L.init_to + dt - L.to)) dt_one = L.init_to + dt - L.to;
 
     const double* yo = L.y;
+    const double to = L.to;  // stage times to + c_i dt_one (dopr54.rs:545-607)
     double yn1[DIM], yerr[DIM], ynk[DIM], a[DIM], k1[DIM], k2[DIM], k3[DIM], k4[DIM], k5[DIM],
         k6[DIM];
 #pragma unroll
@@ -705,13 +708,13 @@ DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F
         yerr[i] = e1 * k1[i];
         ynk[i] = yo[i] + a21 * k1[i];
     }
-    f(ynk, a);
+    f(to + c2 * dt_one, ynk, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
         k2[i] = dt_one * a[i];
         ynk[i] = yo[i] + a31 * k1[i] + a32 * k2[i];
     }
-    f(ynk, a);
+    f(to + c3 * dt_one, ynk, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
         k3[i] = dt_one * a[i];
@@ -719,7 +722,7 @@ DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F
         yerr[i] += e3 * k3[i];
         ynk[i] = yo[i] + a41 * k1[i] + a42 * k2[i] + a43 * k3[i];
     }
-    f(ynk, a);
+    f(to + c4 * dt_one, ynk, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
         k4[i] = dt_one * a[i];
@@ -727,7 +730,7 @@ DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F
         yerr[i] += e4 * k4[i];
         ynk[i] = yo[i] + a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i];
     }
-    f(ynk, a);
+    f(to + c5 * dt_one, ynk, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
         k5[i] = dt_one * a[i];
@@ -735,7 +738,7 @@ DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F
         yerr[i] += e5 * k5[i];
         ynk[i] = yo[i] + a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i];
     }
-    f(ynk, a);
+    f(to + dt_one, ynk, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
         k6[i] = dt_one * a[i];
@@ -743,7 +746,7 @@ DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F
         yerr[i] += e6 * k6[i];
         ynk[i] = yo[i] + a71 * k1[i] + a73 * k3[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i];
     }
-    f(ynk, a);
+    f(to + dt_one, ynk, a);
 #pragma unroll
     for (int i = 0; i < DIM; i++) yerr[i] += e7 * dt_one * a[i];
 

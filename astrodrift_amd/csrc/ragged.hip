@@ -37,12 +37,13 @@ struct LaneForce {
     FT c0, c1, c2;
     const FT* table;
     const PotParams* p;  // far table (MW outside the LDS range)
-    __device__ __forceinline__ void operator()(const double* q, double* a) const {
+    __device__ __forceinline__ void operator()(double t, const double* q, double* a) const {
         switch (kind) {
-            case POT_GALPY_KEPLER: GalpyKeplerForce<FT>{}(q, a); break;
-            case POT_KEPLER: KeplerForce<FT>{c0}(q, a); break;
-            case POT_PLUMMER: PlummerForce<FT>{c0, c1}(q, a); break;
-            default: MwForce<FT>{p, table, c0, c1, c2}(q, a); break;
+            case POT_GALPY_KEPLER: GalpyKeplerForce<FT>{}(t, q, a); break;
+            case POT_KEPLER: KeplerForce<FT>{c0}(t, q, a); break;
+            case POT_PLUMMER: PlummerForce<FT>{c0, c1}(t, q, a); break;
+            case POT_NONE: NoForce{}(t, q, a); break;
+            default: MwForce<FT>{p, table, c0, c1, c2}(t, q, a); break;
         }
     }
 };
@@ -445,9 +446,15 @@ int resolve_pots(drift_session* s, const drift_potential* potentials, size_t npo
     t64.assign(npots, nullptr);
     t32.assign(npots, nullptr);
     for (size_t k = 0; k < npots; ++k) {
+        if (potentials && potentials[k].origins != 0)
+            return fail(DRIFT_ERR_UNSUPPORTED,
+                        "moving origins are supported by the single-batch entry points only");
         int rc = resolve_potential(s, potentials ? &potentials[k] : nullptr, pots[k], &t64[k],
                                    &t32[k]);
         if (rc != DRIFT_OK) return rc;
+        // without origins the only moving kind is NONE's empty set, which the
+        // lane force evaluates as the static NONE kind
+        pots[k].kind = static_kind(pots[k].kind);
     }
     return DRIFT_OK;
 }

@@ -373,6 +373,9 @@ int drift_server_submit(drift_server* S, const drift_request* request, uint64_t*
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "rtol and atol (natural logs) must be finite");
     if (S->opt.precision == DRIFT_PRECISION_MIXED && std::exp(q.rtol) < 1e-7 * (1 - 1e-9))
         return fail(DRIFT_ERR_UNSUPPORTED, "mixed precision requires rtol >= 1e-7");
+    if (q.potential.origins != 0)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "moving origins are supported by the single-batch entry points only");
     int rc = validate_potential(q.potential);
     if (rc != DRIFT_OK) return rc;
     auto r = std::make_shared<Request>();

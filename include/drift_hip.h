@@ -20,7 +20,11 @@
  *   - per-orbit statistics and status (the reference GPU has no error channel);
  *   - a session handle owning the device, stream and cached buffers
  *     (the reference creates a context per launch, gpu/cuda_oxide.rs:16-18);
- *   - device-resident variants that take device pointers and a HIP stream.
+ *   - device-resident variants that take device pointers and a HIP stream;
+ *   - moving origins (point masses and Plummer spheres on tabulated quintic
+ *     tracks), the reference's CustomOrigin (shared/src/potential.rs:264-356),
+ *     whose recipes never construct it (the FIXMEs of shared/src/model.rs:
+ *     193-199, 226-235).
  *
  * All functions return DRIFT_OK (0) or a drift_status code.  Thread safety: a
  * session may be used from one thread at a time; distinct sessions are
@@ -36,7 +40,7 @@
 extern "C" {
 #endif
 
-#define DRIFT_ABI_VERSION 3
+#define DRIFT_ABI_VERSION 4
 
 /* Status codes.  They map onto the reference's error taxonomy
  * (DispatchError, src/dispatch/mod.rs:36-79; IntegrationError,
@@ -61,8 +65,9 @@ typedef enum drift_potential_kind {
     DRIFT_POTENTIAL_GALPY_KEPLER = 0, /* unit point mass, galpy cylindrical form (dopr54.rs:75-88) */
     DRIFT_POTENTIAL_KEPLER = 1,       /* shared/src/potential.rs:217-239 */
     DRIFT_POTENTIAL_PLUMMER = 2,      /* shared/src/potential.rs:241-262 */
-    DRIFT_POTENTIAL_MW = 3            /* bulge + Miyamoto-Nagai disk + NFW halo,
+    DRIFT_POTENTIAL_MW = 3,           /* bulge + Miyamoto-Nagai disk + NFW halo,
                                          shared/src/potential.rs:87-215 */
+    DRIFT_POTENTIAL_NONE = 4          /* no static force: moving origins only, or free motion */
 } drift_potential_kind;
 
 typedef enum drift_precision {
@@ -73,10 +78,14 @@ typedef enum drift_precision {
 
 /* Potential descriptor (G = 1 natural units).  Unused fields are ignored.
  * MW: bulge rho = bulge_amp r^-alpha exp(-(r/rc)^2) (0 <= alpha < 3),
- *     Miyamoto-Nagai (disk_amp, disk_a, disk_b), NFW (halo_amp, halo_a). */
+ *     Miyamoto-Nagai (disk_amp, disk_a, disk_b), NFW (halo_amp, halo_a).
+ * origins: 0, or a moving-origin set of the launching session
+ * (drift_origins_create) whose force is added to the static kind's
+ * (GALPY_KEPLER excepted).  Moving origins need FP64 precision and the
+ * single-batch entry points (not drift_ragged_device or the server). */
 typedef struct drift_potential {
     int32_t kind;      /* drift_potential_kind */
-    int32_t reserved;  /* must be 0 */
+    int32_t origins;   /* moving-origin set id, 0 = none */
     double amp;        /* Kepler / Plummer amplitude */
     double b;          /* Plummer scale radius */
     double bulge_amp, bulge_alpha, bulge_rc;
@@ -313,13 +322,49 @@ int drift_server_wait(drift_server* server, uint64_t ticket, double timeout_s,
 /* Finishes every submitted request, stops the loop and frees the server. */
 int drift_server_destroy(drift_server* server);
 
+/* ---- moving origins (CustomOrigin, shared/src/potential.rs:264-356) ------ */
+
+/* Uploads a set of moving origins to the session and returns its id (> 0) in
+ * *origins_id, for drift_potential.origins.  Origin i is a point mass
+ * (params[2i + 1] = b = 0, KeplerPotential) or a Plummer sphere (b > 0) of
+ * amplitude params[2i] whose centre follows
+ *   o_i(t) = quintic_s(tau),  u = (t - t_start) n_segments / (t_end - t_start),
+ *   s = floor(u) clamped to [0, n_segments - 1],  tau = u - s,
+ * with tracks[((i * n_segments) + s) * 18 + 6 c + k] the tau^(5 - k)
+ * coefficient of coordinate c (x, y, z), the order of CustomOrigin's
+ * QuinticCoeff a..f (potential.rs:273-289).  The force at x is
+ * sum_i F_i(x - o_i(t)), added to the static force.  Host arrays, copied
+ * (synchronously); the set lives until drift_origins_destroy or the end of
+ * the session. */
+int drift_origins_create(drift_session* session, const double* tracks, const double* params,
+                         int32_t n_origins, int32_t n_segments, double t_start, double t_end,
+                         int32_t* origins_id);
+/* Frees a set.  No launch that uses it may still be running. */
+int drift_origins_destroy(drift_session* session, int32_t origins_id);
+
+/* Host-only: quintic Hermite tracks of n orbits sampled at nt strictly
+ * monotone, uniformly spaced times, matching position, velocity and
+ * acceleration at every sample.  states: nt x n x 6 (time-major, as the
+ * integrators write it), accels: nt x n x 3.  Writes n x (nt - 1) x 18
+ * coefficients in drift_origins_create's layout, segments in increasing time,
+ * and the span [*t_start, *t_end].  No device needed. */
+int drift_origin_tracks_hermite(const double* states, const double* accels, const double* times,
+                                size_t nt, size_t n, double* tracks, double* t_start,
+                                double* t_end);
+
 /* ---- force evaluation (test and diagnostics entry) ----------------------- */
 
 /* Accelerations of the device force formulation at n points.
- * d_pos, d_acc: n x 3 doubles on the device. */
+ * d_pos, d_acc: n x 3 doubles on the device; d_t: n times (device), or NULL
+ * for t = 0 (only moving origins depend on time). */
 int drift_force_device(drift_session* session, const drift_potential* potential,
-                       const double* d_pos, double* d_acc, size_t n, int32_t precision,
-                       void* stream);
+                       const double* d_pos, const double* d_t, double* d_acc, size_t n,
+                       int32_t precision, void* stream);
+
+/* The same in FP64 on host buffers (copied through the session's stream,
+ * which is synchronised): pos, acc n x 3, t n times or NULL for t = 0. */
+int drift_force(drift_session* session, const drift_potential* potential, const double* pos,
+                const double* t, double* acc, size_t n);
 
 /* Host-only check of the Milky-Way spherical force tables: worst relative
  * error of the tabulated S(r^2) against its long-double evaluation over

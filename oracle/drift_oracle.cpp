@@ -71,6 +71,7 @@ inline Counted m_pow(Counted x, Counted y) { tally(OP_TRANS); return Counted(::p
 inline Counted m_fabs(Counted x) { return Counted(::fabs(x.v)); }
 inline Counted m_fmax(Counted x, Counted y) { return Counted(::fmax(x.v, y.v)); }
 inline Counted m_ceil(Counted x) { return Counted(::ceil(x.v)); }
+inline Counted m_floor(Counted x) { return Counted(::floor(x.v)); }
 inline Counted m_round(Counted x) { return Counted(::round(x.v)); }
 inline double m_value(Counted x) { return x.v; }
 inline Counted m_lower_gamma(double a, Counted x) {
@@ -180,6 +181,15 @@ void oracle_count_ops(int method, const Potential* pot, const double* y0, const 
 void oracle_force(const Potential* pot, double x, double y, double z, double* out3) {
     double ax, ay, az;
     force<double>(*pot, x, y, z, ax, ay, az);
+    out3[0] = ax;
+    out3[1] = ay;
+    out3[2] = az;
+}
+
+// Static force plus moving origins at time t.
+void oracle_force_at(const Potential* pot, double t, double x, double y, double z, double* out3) {
+    double ax, ay, az;
+    force_at<double>(*pot, t, x, y, z, ax, ay, az);
     out3[0] = ax;
     out3[1] = ay;
     out3[2] = az;

@@ -39,6 +39,7 @@ inline double m_pow(double x, double y) { return ::pow(x, y); }
 inline double m_fabs(double x) { return ::fabs(x); }
 inline double m_fmax(double x, double y) { return ::fmax(x, y); }
 inline double m_ceil(double x) { return ::ceil(x); }
+inline double m_floor(double x) { return ::floor(x); }
 inline double m_round(double x) { return ::round(x); }
 inline bool m_isnan(double x) { return x != x; }
 inline double m_value(double x) { return x; }
@@ -69,6 +70,7 @@ enum PotentialKind : int32_t {
     POT_KEPLER = 1,        // shared/src/potential.rs:217-239
     POT_PLUMMER = 2,       // shared/src/potential.rs:241-262
     POT_MW = 3,            // bulge + Miyamoto-Nagai + NFW, shared/src/potential.rs:87-176
+    POT_NONE = 4,          // no static force (moving origins only)
 };
 
 struct Potential {
@@ -79,6 +81,12 @@ struct Potential {
     double bulge_amp, bulge_alpha, bulge_rc;
     double disk_amp, disk_a, disk_b;
     double halo_amp, halo_a;
+    // Moving origins (CustomOrigin, shared/src/potential.rs:264-356), added
+    // to the static force; n_origins = 0: none.
+    const double* origin_tracks;  // n_origins x n_segments x 18 coefficients
+    const double* origin_params;  // n_origins x (amp, b); b = 0: KeplerPotential
+    int32_t n_origins, n_segments;
+    double t_start, t_end;        // the segments split [t_start, t_end] evenly
 };
 
 // Lower incomplete gamma gamma(a, x) (unnormalised), by the power series
@@ -248,17 +256,89 @@ inline void force(const Potential& p, const R& x, const R& y, const R& z, R& ax,
         case POT_GALPY_KEPLER: galpy_kepler_force(x, y, z, ax, ay, az); break;
         case POT_KEPLER: kepler_force(p.amp, x, y, z, ax, ay, az); break;
         case POT_PLUMMER: plummer_force(p.amp, p.b, x, y, z, ax, ay, az); break;
+        case POT_NONE: ax = ay = az = R(0.0); break;
         default: mw_force(p, x, y, z, ax, ay, az); break;
     }
 }
 
-// Right-hand side in the reference's FuncPtr shape (t, q, a, nargs, args):
-// a = (q3, q4, q5, F(q0, q1, q2)), src/integrators/galpy/dopr54.rs:59-73.
+// quintic_interp, shared/src/potential.rs:282-288 (c = QuinticCoeff a..f).
 template <class R>
-inline void rhs(const Potential& p, R /*t*/, const R* q, R* a) {
+inline R quintic_interp(R t, const double* c) {
+    R t2 = t * t;
+    R t3 = t * t2;
+    R t4 = t * t3;
+    R t5 = t * t4;
+    return c[0] * t5 + c[1] * t4 + c[2] * t3 + c[3] * t2 + c[4] * t + c[5];
+}
+
+// CustomOrigin::origins, shared/src/potential.rs:290-331: origin i's position
+// at time t.  The reference takes segment floor(t / dt), dt = final_time /
+// division (:292-294), evaluates the quintic at t minus the segment INDEX
+// (:327-329) and reads row i * n + segment with n = length / division, the
+// number of origins (:293-295).  Here the tracks start at t_start (the
+// reference's 0), the local time is the segment's own, u - segment with
+// u = (t - t_start) / dt, in [0, 1), origin i's segments follow each other
+// (row i * n_segments + segment), and a time past the last segment
+// extrapolates it where the reference reads past its table.  With t_start = 0,
+// dt = 1 and as many origins as segments the two readings are the same.
+template <class R>
+inline void origin_position(const Potential& p, R t, int32_t i, R* o) {
+    const double dt = (p.t_end - p.t_start) / (double)p.n_segments;
+    R u = (t - p.t_start) / dt;
+    double seg = m_value(m_floor(u));
+    if (!(seg >= 0.0)) seg = 0.0;  // `as usize` saturates negative and NaN to 0
+    if (seg > (double)(p.n_segments - 1)) seg = (double)(p.n_segments - 1);
+    R tau = u - seg;
+    const double* row = p.origin_tracks + ((int64_t)i * p.n_segments + (int64_t)seg) * 18;
+    for (int k = 0; k < 3; k++) o[k] = quintic_interp(tau, row + 6 * k);
+}
+
+// CustomOrigin::force, shared/src/potential.rs:344-356: the origins' forces
+// summed in order, each the point-mass (b = 0) or Plummer force of the
+// position relative to the origin.
+template <class R>
+inline void origin_force(const Potential& p, R t, const R& x, const R& y, const R& z, R& ax,
+                         R& ay, R& az) {
+    R fx = R(0.0), fy = R(0.0), fz = R(0.0);
+    for (int32_t i = 0; i < p.n_origins; i++) {
+        R o[3];
+        origin_position(p, t, i, o);
+        const double amp = p.origin_params[2 * i], b = p.origin_params[2 * i + 1];
+        R gx, gy, gz;
+        if (b == 0.0)
+            kepler_force(amp, x - o[0], y - o[1], z - o[2], gx, gy, gz);
+        else
+            plummer_force(amp, b, x - o[0], y - o[1], z - o[2], gx, gy, gz);
+        fx += gx;
+        fy += gy;
+        fz += gz;
+    }
+    ax = fx;
+    ay = fy;
+    az = fz;
+}
+
+// Static force plus the moving origins' at time t.
+template <class R>
+inline void force_at(const Potential& p, R t, const R& x, const R& y, const R& z, R& ax, R& ay,
+                     R& az) {
+    force(p, x, y, z, ax, ay, az);
+    if (p.n_origins > 0) {
+        R ox, oy, oz;
+        origin_force(p, t, x, y, z, ox, oy, oz);
+        ax = ax + ox;
+        ay = ay + oy;
+        az = az + oz;
+    }
+}
+
+// Right-hand side in the reference's FuncPtr shape (t, q, a, nargs, args):
+// a = (q3, q4, q5, F(t, q0, q1, q2)), src/integrators/galpy/dopr54.rs:59-73.
+template <class R>
+inline void rhs(const Potential& p, R t, const R* q, R* a) {
     RegionGuard<R> guard(REG_RHS);
     R ax, ay, az;
-    force(p, q[0], q[1], q[2], ax, ay, az);
+    force_at(p, t, q[0], q[1], q[2], ax, ay, az);
     a[0] = q[3];
     a[1] = q[4];
     a[2] = q[5];
@@ -276,6 +356,7 @@ inline double potential_value(const Potential& p, double x, double y, double z) 
         case POT_GALPY_KEPLER: return -1.0 / r;
         case POT_KEPLER: return -p.amp / r;
         case POT_PLUMMER: return -p.amp / ::sqrt(r2 + p.b * p.b);
+        case POT_NONE: return 0.0;
         default: break;
     }
     const double two_pi = 6.283185307179586;

@@ -27,6 +27,13 @@ class OPotential(ctypes.Structure):
             "amp", "b", "bulge_amp", "bulge_alpha", "bulge_rc",
             "disk_amp", "disk_a", "disk_b", "halo_amp", "halo_a",
         )
+    ] + [
+        ("origin_tracks", ctypes.c_void_p),
+        ("origin_params", ctypes.c_void_p),
+        ("n_origins", ctypes.c_int32),
+        ("n_segments", ctypes.c_int32),
+        ("t_start", ctypes.c_double),
+        ("t_end", ctypes.c_double),
     ]
 
 
@@ -65,6 +72,9 @@ def load() -> ctypes.CDLL:
                                          c.c_double, c.c_double, c.c_void_p, c.c_void_p]
         lib.oracle_force.restype = None
         lib.oracle_force.argtypes = [P, c.c_double, c.c_double, c.c_double, c.c_void_p]
+        lib.oracle_force_at.restype = None
+        lib.oracle_force_at.argtypes = [P, c.c_double, c.c_double, c.c_double, c.c_double,
+                                        c.c_void_p]
         lib.oracle_potential_value.restype = c.c_double
         lib.oracle_potential_value.argtypes = [P, c.c_double, c.c_double, c.c_double]
         lib.oracle_lower_gamma.restype = c.c_double
@@ -80,12 +90,25 @@ def load() -> ctypes.CDLL:
 
 
 def to_opot(spec) -> OPotential:
-    """From astrodrift_amd.PotentialSpec (or None = galpy unit Kepler)."""
+    """From astrodrift_amd.PotentialSpec (or None = galpy unit Kepler),
+    moving origins included."""
     if spec is None:
         return OPotential(kind=0)
-    return OPotential(spec.kind, 0, spec.amp, spec.b, spec.bulge_amp, spec.bulge_alpha,
-                      spec.bulge_rc, spec.disk_amp, spec.disk_a, spec.disk_b, spec.halo_amp,
-                      spec.halo_a)
+    pot = OPotential(spec.kind, 0, spec.amp, spec.b, spec.bulge_amp, spec.bulge_alpha,
+                     spec.bulge_rc, spec.disk_amp, spec.disk_a, spec.disk_b, spec.halo_amp,
+                     spec.halo_a)
+    o = getattr(spec, "origins", None)
+    if o is not None:
+        tracks = np.ascontiguousarray(o.tracks, dtype=np.float64)
+        params = np.ascontiguousarray(o.params, dtype=np.float64)
+        pot.origin_tracks = tracks.ctypes.data
+        pot.origin_params = params.ctypes.data
+        pot.n_origins = tracks.shape[0]
+        pot.n_segments = tracks.shape[1]
+        pot.t_start = o.t_start
+        pot.t_end = o.t_end
+        pot._keep = (tracks, params)  # the arrays the pointers refer to
+    return pot
 
 
 def _p(a: np.ndarray) -> ctypes.c_void_p:
@@ -139,6 +162,15 @@ def force(spec, x, y, z):
     out = np.zeros(3)
     pot = to_opot(spec)
     lib.oracle_force(ctypes.byref(pot), x, y, z, _p(out))
+    return out
+
+
+def force_at(spec, t, x, y, z):
+    """Static force plus the moving origins' at time t."""
+    lib = load()
+    out = np.zeros(3)
+    pot = to_opot(spec)
+    lib.oracle_force_at(ctypes.byref(pot), t, x, y, z, _p(out))
     return out
 
 

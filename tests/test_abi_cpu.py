@@ -23,7 +23,8 @@ def test_header_declares_the_expected_entry_points():
         "drift_get_abi_version", "drift_last_error", "drift_session_create",
         "drift_session_destroy", "drift_galpy_dopr54", "drift_galpy_dop853",
         "drift_galpy_dopr54_device", "drift_galpy_dop853_device", "drift_force_device",
-        "drift_mw_table_check",
+        "drift_mw_table_check", "drift_origins_create", "drift_origins_destroy",
+        "drift_origin_tracks_hermite",
     } <= names
 
 
@@ -36,7 +37,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_layouts():
     lib = _native.load_library()
-    assert lib.drift_get_abi_version() == _native.ABI_VERSION == 3
+    assert lib.drift_get_abi_version() == _native.ABI_VERSION == 4
     assert ctypes.sizeof(_native.CPotential) == 8 + 10 * 8
     assert ctypes.sizeof(_native.COrbitStats) == 16
     assert ctypes.sizeof(_native.CLaunchOptions) == 24

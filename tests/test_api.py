@@ -248,10 +248,41 @@ def test_model_force_resolution():
     sim.add(node, dft.background(k))
     spec = sim._force_for(node)
     assert spec.kind == 2 and spec.amp == 2.0 and spec.b == 0.3
-    sim2 = dft.Config(engine=dft.Engine.GPU, forces="model")
-    sim2.add(node, dft.particles(k, np.zeros((1, 6))))
-    with pytest.raises(NotImplementedError, match="moving-origin"):
-        sim2._force_for(node)
     assert dft.Config(engine=dft.Engine.GPU)._force_for(node) is None
     with pytest.raises(ValueError):
         dft.Config(forces="bogus")
+
+
+def test_model_force_resolution_of_several_backgrounds():
+    sim = dft.Config(engine=dft.Engine.GPU, forces="model", ts=(0.0, 4.0, 5))
+    node = dft.test_particles(np.zeros((1, 6)))
+    sim.add(node, dft.background(dft.Potential.bovy()), dft.background(dft.Potential.kepler(0.5)),
+            dft.background(dft.Potential.plummer(0.2, 0.1)))
+    spec = sim._force_for(node)
+    # the bovy model is the static force; the point masses become origins at rest
+    assert spec.kind == 3 and spec.origins is not None
+    np.testing.assert_array_equal(spec.origins.params, [[0.5, 0.0], [0.2, 0.1]])
+    assert not spec.origins.tracks.any()
+    assert (spec.origins.t_start, spec.origins.t_end, spec.origins.n_segments) == (0.0, 4.0, 4)
+    lone = dft.Config(engine=dft.Engine.GPU, forces="model")
+    free = dft.test_particles(np.zeros((1, 6)))
+    lone.add(free, dft.test_particles(np.zeros((1, 6))))
+    assert lone._force_for(free).kind == 4  # no potential among the sources: free motion
+    two = dft.Config(engine=dft.Engine.GPU, forces="model")
+    two.add(free, dft.background(dft.Potential.bovy()), dft.background(dft.Potential.bovy()))
+    with pytest.raises(NotImplementedError, match="at most one bovy"):
+        two._force_for(free)
+
+
+def test_model_integration_order_puts_force_sources_first():
+    sim = dft.Config(engine=dft.Engine.GPU, forces="model")
+    bg = dft.background(dft.Potential.bovy())
+    stream = dft.test_particles(np.zeros((2, 6)))
+    sat = dft.particles(dft.Potential.plummer(1e-3, 0.01), np.zeros((1, 6)))
+    sim.add(stream, bg, sat)
+    sim.add(sat, bg)
+    assert [c._identity for c in sim._integration_order()] == [sat._identity, stream._identity]
+    ref = dft.Config(engine=dft.Engine.GPU)
+    ref.add(stream, bg, sat)
+    ref.add(sat, bg)
+    assert [c._identity for c in ref._integration_order()] == [stream._identity, sat._identity]
