@@ -11,6 +11,10 @@ all deliberate:
 * the per-container particle limit (MAX_PARTICLES = 1000,
   shared/src/lib.rs:25) is lifted, and the output-time limit is raised from
   1024 to MAX_OUTPUT_TIMES;
+* ``Config(..., arbitrary_ts=True)`` takes output times that are not
+  uniformly spaced and passes them to the integrators as given (the
+  reference's FIXME, src/interface/mod.rs:89-90); DOPR54, which steps the
+  fixed interval t[1] - t[0], refuses such grids;
 * ``Engine.CPU`` configurations validate as before but ``run()`` raises
   NotImplementedError: this package replaces the GPU engine only;
 * ``Config(..., forces="model")`` integrates each container in the potential
@@ -73,9 +77,14 @@ class OutputGrid:
     start: float = 0.0
     end: float = 2.0 * math.pi
     steps: int = 100
+    # arbitrary requested times (Config(..., arbitrary_ts=True)), passed to the
+    # integrators as given; None for the affine grid
+    explicit: tuple | None = None
 
     def sample(self, index: int) -> float:
         """OutputGrid::sample, shared/src/config.rs:57-66."""
+        if self.explicit is not None:
+            return self.explicit[index]
         if index == self.steps - 1:
             return self.end
         return self.start + (self.end - self.start) * float(index) / float(self.steps - 1)
@@ -83,6 +92,20 @@ class OutputGrid:
     def times(self) -> np.ndarray:
         """sample_times, src/dispatch/mod.rs:14-18."""
         return np.array([self.sample(i) for i in range(self.steps)], dtype=np.float64)
+
+    def is_affine(self) -> bool:
+        """Whether the times are the affine grid's within GRID_ULP_TOLERANCE
+        (src/interface/mod.rs:110-131)."""
+        if self.explicit is None:
+            return True
+        affine = OutputGrid(self.start, self.end, self.steps)
+        step = (self.end - self.start) / float(self.steps - 1)
+        for index, actual in enumerate(self.explicit):
+            expected = affine.sample(index)
+            scale = max(abs(actual), abs(expected), abs(step), _MIN_POSITIVE)
+            if abs(actual - expected) > GRID_ULP_TOLERANCE * _EPS * scale:
+                return False
+        return True
 
 
 def _grid_new(start: float, end: float, steps: int) -> OutputGrid:
@@ -96,7 +119,7 @@ def _grid_new(start: float, end: float, steps: int) -> OutputGrid:
     return OutputGrid(float(start), float(end), int(steps))
 
 
-def _grid_from_times(times: Sequence[float]) -> OutputGrid:
+def _grid_from_times(times: Sequence[float], arbitrary: bool = False) -> OutputGrid:
     times = [float(t) for t in times]
     if not (2 <= len(times) <= MAX_OUTPUT_TIMES):
         raise ValueError(f"ts must contain between 2 and {MAX_OUTPUT_TIMES} output times")
@@ -107,19 +130,19 @@ def _grid_from_times(times: Sequence[float]) -> OutputGrid:
     for a, b in zip(times, times[1:]):
         if (b <= a) if increasing else (b >= a):
             raise ValueError("ts values must be strictly increasing or strictly decreasing")
-    step = (grid.end - grid.start) / float(grid.steps - 1)
-    for index, actual in enumerate(times):
-        expected = grid.sample(index)
-        scale = max(abs(actual), abs(expected), abs(step), _MIN_POSITIVE)
-        if abs(actual - expected) > GRID_ULP_TOLERANCE * _EPS * scale:
-            raise ValueError(
-                "ts values must be uniformly spaced; arbitrary output times are not yet supported"
-            )
+    if arbitrary:
+        # the reference's FIXME (src/interface/mod.rs:89-90): keep the times as given
+        return OutputGrid(grid.start, grid.end, grid.steps, tuple(times))
+    if not OutputGrid(grid.start, grid.end, grid.steps, tuple(times)).is_affine():
+        raise ValueError(
+            "ts values must be uniformly spaced; arbitrary output times are not yet supported"
+        )
     return grid
 
 
-def _parse_grid(obj: Any) -> OutputGrid:
-    """PyOutputGrid::extract, src/interface/mod.rs:31-65."""
+def _parse_grid(obj: Any, arbitrary: bool = False) -> OutputGrid:
+    """PyOutputGrid::extract, src/interface/mod.rs:31-65.  With `arbitrary`,
+    a sequence of times need not be uniformly spaced."""
     if isinstance(obj, tuple):
         if len(obj) != 3:
             raise ValueError("time-grid tuple must have 3 elements: (start, end, steps)")
@@ -132,14 +155,14 @@ def _parse_grid(obj: Any) -> OutputGrid:
     if isinstance(obj, np.ndarray) and obj.dtype == np.float64 and obj.ndim == 1:
         if not obj.flags.c_contiguous:
             raise ValueError("NumPy time arrays must be contiguous")
-        return _grid_from_times(obj.tolist())
+        return _grid_from_times(obj.tolist(), arbitrary)
     try:
         seq = [float(v) for v in obj]
     except TypeError:
         raise ValueError(
             "Expected (start, end, steps) tuple or 1D numpy.linspace array"
         ) from None
-    return _grid_from_times(seq)
+    return _grid_from_times(seq, arbitrary)
 
 
 def _parse_tolerance(obj: Any) -> tuple[float, float]:
@@ -359,12 +382,13 @@ class Config:
         tolerance: Any = None,
         *,
         forces: str = "reference",
+        arbitrary_ts: bool = False,
         device: int = 0,
     ):
         self.engine = _selector(engine, Engine, "Engine")
         self.method = _selector(method, Method, "Method")
         self.implementation = _selector(implementation, Implementation, "Implementation")
-        self.grid = OutputGrid() if ts is None else _parse_grid(ts)
+        self.grid = OutputGrid() if ts is None else _parse_grid(ts, bool(arbitrary_ts))
         self.rtol, self.atol = (MIN_RTOL, MIN_ATOL) if tolerance is None else _parse_tolerance(tolerance)
         if forces not in ("reference", "model"):
             raise ValueError("forces must be 'reference' or 'model'")
@@ -528,6 +552,11 @@ class Config:
         if self.engine is Engine.CPU:
             raise NotImplementedError(
                 "astrodrift_amd provides the GPU engine only; use Engine.GPU"
+            )
+        if self.method is Method.DOPR54 and not self.grid.is_affine():
+            raise NotImplementedError(
+                "arbitrary output times need Method.DOP853: DOPR54 steps the fixed interval "
+                "t[1] - t[0] (src/integrators/galpy/dopr54.rs:713)"
             )
         times = self.grid.times()
         rtol_ln, atol_ln = log_tolerance(self.rtol, self.atol)

@@ -398,10 +398,18 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
         abandon_orbit(L.ii, J.nt, L.status, STATUS_NONFINITE, sink);
         return true;
     }
-    const double fac11 = std::sqrt(std::sqrt(std::sqrt(err)));  // err^(1/8)
-    constexpr double inv_safe = 1.0 / safe;
-    const double fac = tmax(facc2, tmin(facc1, fac11 * inv_safe));
-    double hnew = h * rcp_fast(fac);
+    // Step factors (dop853.rs:621-623, :775) as multipliers of h:
+    //   accept: h / max(facc2, min(facc1, fac11 / safe)) = h min(1/facc2, shrink)
+    //   reject: h / min(facc1, fac11 / safe)            = h shrink
+    // with shrink = max(1/facc1, safe err^(-1/8)) and fac11 = err^(1/8).
+    // err^(-1/8) is three reciprocal square roots; err = 0 (or below the
+    // normal range) gives an infinite factor, err = inf a zero one, as the
+    // reference's limits of pow(err, 1/8).
+    const double root8 = rsqrt_pos(rsqrt_pos(rsqrt_pos(err)));
+    const double grow = err > 1e-300 ? (err < INFINITY ? safe * root8 : 0.0) : INFINITY;
+    constexpr double inv_facc1 = 1.0 / facc1, inv_facc2 = 1.0 / facc2;
+    const double shrink = tmax(inv_facc1, grow);
+    double hnew = h * tmin(inv_facc2, shrink);
 
     if (err <= 1.0) {
         L.accepted++;
@@ -514,7 +522,7 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
         L.t_cur = t_new;
     } else {
         L.rejected++;
-        hnew = h * rcp_fast(tmin(facc1, fac11 * inv_safe));
+        hnew = h * shrink;
         L.reject = 1;
         // t_current = t_old; t_old = t_old_older (dop853.rs:777-778)
         L.t_cur = t_step;

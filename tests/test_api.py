@@ -286,3 +286,28 @@ def test_model_integration_order_puts_force_sources_first():
     ref.add(stream, bg, sat)
     ref.add(sat, bg)
     assert [c._identity for c in ref._integration_order()] == [stream._identity, sat._identity]
+
+
+def test_arbitrary_times_are_opt_in():
+    times = np.logspace(-1.0, 1.0, 9)
+    with pytest.raises(ValueError, match="uniformly spaced"):
+        dft.Config(ts=times)
+    cfg = dft.Config(ts=times, arbitrary_ts=True)
+    np.testing.assert_array_equal(cfg.grid.times(), times)
+    assert not cfg.grid.is_affine()
+    assert dft.Config(ts=np.linspace(0.0, 1.0, 7), arbitrary_ts=True).grid.is_affine()
+    back = dft.Config(ts=[3.0, 2.5, 0.5, -1.0], arbitrary_ts=True)
+    np.testing.assert_array_equal(back.grid.times(), [3.0, 2.5, 0.5, -1.0])
+    for bad, message in (([0.0, 0.5, 0.25, 1.0], "strictly"), ([0.0, float("inf")], "finite"),
+                         ([1.0], "between 2")):
+        with pytest.raises(ValueError, match=message):
+            dft.Config(ts=bad, arbitrary_ts=True)
+
+
+def test_dopr54_refuses_arbitrary_times_before_launching():
+    sim = dft.Config(engine=dft.Engine.GPU, method=dft.Method.DOPR54,
+                     ts=[0.0, 0.1, 0.5, 2.0], arbitrary_ts=True)
+    sim.add(dft.test_particles(np.array([[1.0, 0, 0, 0, 1.0, 0]])),
+            dft.background(dft.Potential.kepler(1.0)))
+    with pytest.raises(NotImplementedError, match="DOP853"):
+        sim.run()

@@ -279,3 +279,22 @@ def mw_force_independent(spec, x, y, z):
     f = -ld(spec.disk_amp) / D3
     return np.array([float(f * x - S * x), float(f * y - S * y),
                      float(f * z * (ld(spec.disk_a) + zb) / zb - S * z)])
+
+
+def test_dop853_arbitrary_times_through_the_api(gpu):
+    """Config(arbitrary_ts=True): log-spaced output times, passed to the
+    kernels as given, against the oracle at the same times."""
+    times = np.concatenate([[0.0], np.logspace(-2.0, 1.5, 60)])
+    y0 = workloads.mw_disk().state0[::8192][:128]
+    cfg = dft.Config(engine=dft.Engine.GPU, method=dft.Method.DOP853, ts=times,
+                     tolerance=1e-10, arbitrary_ts=True, forces="model")
+    cfg.add(dft.test_particles(y0), dft.background(dft.Potential.bovy()))
+    print("arbitrary times: 128 MW orbits, 61 log-spaced samples", flush=True)
+    out, none = cfg.run()
+    assert none is None
+    tol = math.log(1e-10)
+    cpu, _ = oracle_lib.batch(_native.METHOD_DOP853, y0, times, tol, tol,
+                              dft.Potential.bovy().spec(), nthreads=8)
+    ep, ev = rel_err(out, cpu)
+    print(f"  pos_rel={ep:.3e} vel_rel={ev:.3e}")
+    assert ep < REL_TOL and ev < REL_TOL
