@@ -30,12 +30,17 @@ namespace drift {
 // ---------------------------------------------------------------------------
 
 // Force of a lane's current job, dispatched at run time (jobs in one launch
-// may use different potentials).  c0..c2 are the kind's constants.
+// may use different potentials).  c0..c2 are the kind's constants.  The MW
+// table of the launch's most used MW potential is staged in LDS (`lds`,
+// in_lds set for its jobs); other MW potentials read theirs from global
+// memory.
 template <class FT>
 struct LaneForce {
     int32_t kind;
+    bool in_lds;
     FT c0, c1, c2;
     const FT* table;
+    const FT* lds;
     const PotParams* p;  // far table (MW outside the LDS range)
     __device__ __forceinline__ void operator()(double t, const double* q, double* a) const {
         switch (kind) {
@@ -43,16 +48,22 @@ struct LaneForce {
             case POT_KEPLER: KeplerForce<FT>{c0}(t, q, a); break;
             case POT_PLUMMER: PlummerForce<FT>{c0, c1}(t, q, a); break;
             case POT_NONE: NoForce{}(t, q, a); break;
-            default: MwForce<FT>{p, table, c0, c1, c2}(t, q, a); break;
+            default:
+                if (in_lds)
+                    MwForce<FT>{p, lds, c0, c1, c2}(t, q, a);
+                else
+                    MwForce<FT>{p, table, c0, c1, c2}(t, q, a);
+                break;
         }
     }
 };
 
+// Sets f to job potential `pot` (f.lds stays the block's LDS table).
 template <class FT>
-__device__ __forceinline__ LaneForce<FT> lane_force(const RaggedArgs& a, int32_t pot) {
+__device__ __forceinline__ void set_lane_force(LaneForce<FT>& f, const RaggedArgs& a, int32_t pot) {
     const PotParams* p = a.pots + pot;
-    LaneForce<FT> f;
     f.kind = p->kind;
+    f.in_lds = pot == a.lds_pot;
     f.p = p;
     if (p->kind == POT_MW) {
         f.c0 = FT(p->disk_amp);
@@ -67,6 +78,20 @@ __device__ __forceinline__ LaneForce<FT> lane_force(const RaggedArgs& a, int32_t
         f.table = static_cast<const FT*>(a.tables64[pot]);
     else
         f.table = static_cast<const FT*>(a.tables32[pot]);
+}
+
+// Stages the launch's LDS table (a no-op without one) and returns a lane force
+// bound to it.
+template <class FT>
+__device__ __forceinline__ LaneForce<FT> stage_lane_table(const RaggedArgs& a, FT* lds) {
+    if (a.lds_pot >= 0) {
+        constexpr int words = sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS;
+        const FT* g = static_cast<const FT*>(a.lds_src);
+        for (int k = threadIdx.x; k < words; k += blockDim.x) lds[k] = g[k];
+        __syncthreads();
+    }
+    LaneForce<FT> f{};
+    f.lds = lds;
     return f;
 }
 
@@ -201,9 +226,10 @@ __device__ __forceinline__ bool claim_item(bool need, unsigned long long* queue,
 
 template <class FT>
 __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dop853_kernel(RaggedArgs a) {
+    __shared__ FT lds[sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS];
+    LaneForce<FT> f = stage_lane_table<FT>(a, lds);
     Dop853Lane L;
     Dop853Job J;
-    LaneForce<FT> f;
     OutSinkR sink{nullptr, 0, 0};
     Resident res{nullptr, 0, 0};
     drift_orbit_stats* stats = nullptr;
@@ -222,7 +248,7 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dop853_kernel(R
             if (phase == kOrbitDone) continue;
             J = make_dop853_job(d.times, d.nt, d.rtol, d.atol, a.max_attempts, d.times[0],
                                 d.times[d.nt - 1]);
-            f = lane_force<FT>(a, d.pot);
+            set_lane_force<FT>(f, a, d.pot);
             sink = OutSinkR{d.out, d.n, orbit};
             stats = d.stats;
             done_count = d.done_count;
@@ -260,9 +286,10 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dop853_kernel(R
 
 template <class FT>
 __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dopr54_kernel(RaggedArgs a) {
+    __shared__ FT lds[sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS];
+    LaneForce<FT> f = stage_lane_table<FT>(a, lds);
     Dopr54Lane L;
     Dopr54Job J;
-    LaneForce<FT> f;
     OutSinkR sink{nullptr, 0, 0};
     Resident res{nullptr, 0, 0};
     drift_orbit_stats* stats = nullptr;
@@ -289,7 +316,7 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dopr54_kernel(R
             J.rtol = d.rtol;
             J.atol = d.atol;
             J.dt_one_init = d.dt_one_init;
-            f = lane_force<FT>(a, d.pot);
+            set_lane_force<FT>(f, a, d.pot);
             sink = OutSinkR{d.out, d.n, orbit};
             stats = d.stats;
             done_count = d.done_count;
@@ -377,9 +404,22 @@ int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, i
     a.queue = scratch.queue;
     a.max_attempts = opt.max_attempts ? opt.max_attempts : (1u << 22);
     dopr54_bounds(a.bound2);
+    const bool mixed = opt.precision == DRIFT_PRECISION_MIXED;
+    // the MW potential with the most orbits in this launch has its table in LDS
+    a.lds_pot = -1;
+    {
+        std::vector<int64_t> orbits(pots.size(), 0);
+        for (int32_t k = 0; k < njobs; ++k) orbits[host_jobs[k].pot] += host_jobs[k].n;
+        int64_t best = 0;
+        for (size_t k = 0; k < pots.size(); ++k)
+            if (pots[k].kind == POT_MW && orbits[k] > best) {
+                best = orbits[k];
+                a.lds_pot = (int32_t)k;
+            }
+        if (a.lds_pot >= 0) a.lds_src = mixed ? t32[a.lds_pot] : t64[a.lds_pot];
+    }
 
     using K = void (*)(RaggedArgs);
-    const bool mixed = opt.precision == DRIFT_PRECISION_MIXED;
     K k;
     if (method == DRIFT_METHOD_DOP853)
         k = mixed ? ragged_dop853_kernel<float> : ragged_dop853_kernel<double>;

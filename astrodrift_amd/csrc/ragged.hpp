@@ -48,6 +48,8 @@ struct RaggedArgs {
     const void* const* tables32;
     unsigned long long* queue;
     double bound2[6];
+    int32_t lds_pot;       // potential whose MW table is staged in LDS, -1 = none
+    const void* lds_src;   // that table in global memory (FP64 or the mixed mode's FP32)
 };
 
 struct RaggedScratch {
