@@ -36,6 +36,7 @@
 
 #include "drift_device.hpp"
 #include "dop853_rkn.hpp"
+#include "dopr54_rkn.hpp"
 
 namespace drift {
 
@@ -588,30 +589,6 @@ inline void dopr54_bounds(double* bound2) {
     }
 }
 
-// rk4_onestep, dopr54.rs:135-208.
-template <class F>
-DRIFT_HD DRIFT_FI void rk4_onestep(const double* yn, double* yn1, double tn, double dt, double* ynk,
-                                   double* a, const F& f) {
-    f(tn, yn, a);
-#pragma unroll
-    for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 6.0;
-#pragma unroll
-    for (int i = 0; i < DIM; i++) ynk[i] = yn[i] + dt * a[i] / 2.0;
-    f(tn + dt / 2.0, ynk, a);
-#pragma unroll
-    for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 3.0;
-#pragma unroll
-    for (int i = 0; i < DIM; i++) ynk[i] = yn[i] + dt * a[i] / 2.0;
-    f(tn + dt / 2.0, ynk, a);
-#pragma unroll
-    for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 3.0;
-#pragma unroll
-    for (int i = 0; i < DIM; i++) ynk[i] = yn[i] + dt * a[i];
-    f(tn + dt, ynk, a);
-#pragma unroll
-    for (int i = 0; i < DIM; i++) yn1[i] += dt * a[i] / 6.0;
-}
-
 DRIFT_HD DRIFT_FI double max_abs6(const double* y) {
     double m = std::fabs(y[0]);
 #pragma unroll
@@ -620,24 +597,54 @@ DRIFT_HD DRIFT_FI double max_abs6(const double* y) {
 }
 
 // rk4_estimate_step, dopr54.rs:210-368, with the scale e^s = atol + rtol max|y|.
+// Each step-doubling test takes three rk4_onestep calls (dopr54.rs:135-208):
+// dt from yo, dt/2 from yo, dt/2 from that half step.  Their 12 stages run as
+// one loop, so the force is inlined once: the estimate runs once per orbit
+// but its register demand sets the whole kernel's.  The arithmetic and its
+// order are rk4_onestep's: stage k of a step of length h from `base` at tn is
+// taken at tn, tn + h/2, tn + h/2, tn + h, adds h a / (6, 3, 3, 6) to the
+// step's accumulator, and the next stage point is base + (h a) (1/2, 1/2, 1).
 template <class F>
 DRIFT_HD DRIFT_FI double rk4_estimate_step(const double* yo, double dt, double to,
                                            const Dopr54Job& J, const F& f) {
     const double init_dt = dt;
     const double sc = J.atol + J.rtol * max_abs6(yo);
-    double yn[DIM], y1[DIM], y21[DIM], y2[DIM], ynk[DIM], a[DIM];
     for (int iter = 0; iter < 64; ++iter) {
+        double y1[DIM], y21[DIM], base[DIM], acc[DIM], ynk[DIM], a[DIM];
+#pragma unroll 1
+        for (int s = 0; s < 12; ++s) {
+            const int o = s >> 2, k = s & 3;  // step o (full, half, second half), stage k
+            if (k == 0) {
 #pragma unroll
-        for (int i = 0; i < DIM; i++) yn[i] = y1[i] = y21[i] = yo[i];
-        rk4_onestep(yn, y1, to, dt, ynk, a, f);
-        rk4_onestep(yn, y21, to, dt / 2.0, ynk, a, f);
+                for (int i = 0; i < DIM; i++) {
+                    base[i] = o == 2 ? y21[i] : yo[i];
+                    acc[i] = base[i];
+                    ynk[i] = base[i];
+                }
+            }
+            const double h = o == 0 ? dt : dt / 2.0;
+            const double tn = o == 2 ? to + dt / 2.0 : to;
+            f(k == 0 ? tn : (k == 3 ? tn + h : tn + h / 2.0), ynk, a);
+            const double w = (k == 0 || k == 3) ? 6.0 : 3.0;
+            const double m = k == 2 ? 1.0 : 0.5;
 #pragma unroll
-        for (int i = 0; i < DIM; i++) y2[i] = y21[i];
-        rk4_onestep(y21, y2, to + dt / 2.0, dt / 2.0, ynk, a, f);
-        double e = 0.0;
+            for (int i = 0; i < DIM; i++) {
+                acc[i] += h * a[i] / w;
+                ynk[i] = base[i] + h * a[i] * m;
+            }
+            if (k == 3 && o == 0) {
+#pragma unroll
+                for (int i = 0; i < DIM; i++) y1[i] = acc[i];
+            }
+            if (k == 3 && o == 1) {
+#pragma unroll
+                for (int i = 0; i < DIM; i++) y21[i] = acc[i];
+            }
+        }
+        double e = 0.0;  // acc holds y2, the two half steps
 #pragma unroll
         for (int i = 0; i < DIM; i++) {
-            double q = (y1[i] - y2[i]) / sc;
+            double q = (y1[i] - acc[i]) / sc;
             e += q * q;
         }
         double err = std::sqrt(e / (double)DIM);
@@ -691,9 +698,17 @@ DRIFT_HD DRIFT_FI void dopr54_init(Dopr54Lane& L, const double* y0, double t0, c
 
 // One pass of the dopr54_onestep loop body (dopr54.rs:403-459) with
 // dopr54_actualstep (:462-659).  Returns true when the orbit is complete.
+//
+// As for DOP853, the right-hand side is (v, F(x)): the stage derivatives'
+// velocity halves are the stage velocities, themselves v0 plus dt times
+// combinations of earlier forces, so the stage positions, the update and the
+// error estimate are formed from the forces F_1..F_7 alone with the combined
+// coefficients of dopr54_rkn.hpp (the same values up to rounding; stage 7's
+// point is the update itself, as a_7j = b_j).
 template <class F, class Sink>
 DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F& f, Sink& sink) {
     using namespace dp54;
+    using namespace rkn54;
     const double dt = J.dt;
     bool accept = false;
     double dt_one = L.dt_one;
@@ -707,56 +722,60 @@ DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F
 
     const double* yo = L.y;
     const double to = L.to;  // stage times to + c_i dt_one (dopr54.rs:545-607)
-    double yn1[DIM], yerr[DIM], ynk[DIM], a[DIM], k1[DIM], k2[DIM], k3[DIM], k4[DIM], k5[DIM],
-        k6[DIM];
+    const double h = dt_one, hh = dt_one * dt_one;
+    double F1[3], F2[3], F3[3], F4[3], F5[3], F6[3], F7[3];
+    double q[DIM], a[DIM];
 #pragma unroll
-    for (int i = 0; i < DIM; i++) {
-        k1[i] = dt_one * L.a1[i];
-        yn1[i] = yo[i] + b1 * k1[i];
-        yerr[i] = e1 * k1[i];
-        ynk[i] = yo[i] + a21 * k1[i];
+    for (int c = 0; c < 3; c++) {
+        F1[c] = L.a1[3 + c];
+        q[3 + c] = 0.0;  // the forces read positions only
     }
-    f(to + c2 * dt_one, ynk, a);
 #pragma unroll
-    for (int i = 0; i < DIM; i++) {
-        k2[i] = dt_one * a[i];
-        ynk[i] = yo[i] + a31 * k1[i] + a32 * k2[i];
+    for (int c = 0; c < 3; c++) q[c] = fma(h * c2, yo[3 + c], yo[c]);
+    f(to + c2 * dt_one, q, a);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        F2[c] = a[3 + c];
+        q[c] = fma(hh, q31 * F1[c], fma(h * c3, yo[3 + c], yo[c]));
     }
-    f(to + c3 * dt_one, ynk, a);
+    f(to + c3 * dt_one, q, a);
 #pragma unroll
-    for (int i = 0; i < DIM; i++) {
-        k3[i] = dt_one * a[i];
-        yn1[i] += b3 * k3[i];
-        yerr[i] += e3 * k3[i];
-        ynk[i] = yo[i] + a41 * k1[i] + a42 * k2[i] + a43 * k3[i];
+    for (int c = 0; c < 3; c++) {
+        F3[c] = a[3 + c];
+        q[c] = fma(hh, q41 * F1[c] + q42 * F2[c], fma(h * c4, yo[3 + c], yo[c]));
     }
-    f(to + c4 * dt_one, ynk, a);
+    f(to + c4 * dt_one, q, a);
 #pragma unroll
-    for (int i = 0; i < DIM; i++) {
-        k4[i] = dt_one * a[i];
-        yn1[i] += b4 * k4[i];
-        yerr[i] += e4 * k4[i];
-        ynk[i] = yo[i] + a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i];
+    for (int c = 0; c < 3; c++) {
+        F4[c] = a[3 + c];
+        q[c] = fma(hh, q51 * F1[c] + q52 * F2[c] + q53 * F3[c], fma(h * c5, yo[3 + c], yo[c]));
     }
-    f(to + c5 * dt_one, ynk, a);
+    f(to + c5 * dt_one, q, a);
 #pragma unroll
-    for (int i = 0; i < DIM; i++) {
-        k5[i] = dt_one * a[i];
-        yn1[i] += b5 * k5[i];
-        yerr[i] += e5 * k5[i];
-        ynk[i] = yo[i] + a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i];
+    for (int c = 0; c < 3; c++) {
+        F5[c] = a[3 + c];
+        q[c] = fma(hh, q61 * F1[c] + q62 * F2[c] + q63 * F3[c] + q64 * F4[c],
+                   fma(h, yo[3 + c], yo[c]));
     }
-    f(to + dt_one, ynk, a);
+    f(to + dt_one, q, a);
+    // 5th-order update (x1, v1); stage 7 is evaluated at it (FSAL)
+    double yn1[DIM], yerr[DIM];
 #pragma unroll
-    for (int i = 0; i < DIM; i++) {
-        k6[i] = dt_one * a[i];
-        yn1[i] += b6 * k6[i];
-        yerr[i] += e6 * k6[i];
-        ynk[i] = yo[i] + a71 * k1[i] + a73 * k3[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i];
+    for (int c = 0; c < 3; c++) {
+        F6[c] = a[3 + c];
+        yn1[c] = fma(hh, bq1 * F1[c] + bq3 * F3[c] + bq4 * F4[c] + bq5 * F5[c],
+                     fma(h, yo[3 + c], yo[c]));
+        yn1[3 + c] = fma(h, b1 * F1[c] + b3 * F3[c] + b4 * F4[c] + b5 * F5[c] + b6 * F6[c],
+                         yo[3 + c]);
     }
-    f(to + dt_one, ynk, a);
+    f(to + dt_one, yn1, a);
 #pragma unroll
-    for (int i = 0; i < DIM; i++) yerr[i] += e7 * dt_one * a[i];
+    for (int c = 0; c < 3; c++) {
+        F7[c] = a[3 + c];
+        yerr[c] = hh * (eq1 * F1[c] + eq3 * F3[c] + eq4 * F4[c] + eq5 * F5[c] + eq6 * F6[c]);
+        yerr[3 + c] = h * (e1 * F1[c] + e3 * F3[c] + e4 * F4[c] + e5 * F5[c] + e6 * F6[c] +
+                           e7 * F7[c]);
+    }
 
     // error norm: mean((yerr / (atol + rtol max|yo|))^2)
     const double maxabs = max_abs6(yo);
@@ -781,7 +800,7 @@ DRIFT_HD DRIFT_FI bool dopr54_attempt(Dopr54Lane& L, const Dopr54Job& J, const F
     if (powertwo >= 0.0 || accept) {
 #pragma unroll
         for (int i = 0; i < DIM; i++) {
-            L.a1[i] = a[i];
+            L.a1[i] = a[i];  // (v1, F7)
             L.y[i] = yn1[i];
         }
         L.to += dt_one;

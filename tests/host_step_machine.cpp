@@ -1,6 +1,6 @@
-// Host build of the product's DOP853 step machine (astrodrift_amd/csrc/
-// integrators.hpp with the forces of forces.hpp), for CPU tests of its
-// arithmetic against the oracle.  Test infrastructure: the library itself
+// Host build of the product's DOP853 and DOPR54 step machines
+// (astrodrift_amd/csrc/integrators.hpp with the forces of forces.hpp), for CPU
+// tests of their arithmetic against the oracle.  Test infrastructure: the library itself
 // only runs these functions inside its gfx950 kernels.
 #include <cstdint>
 #include <vector>
@@ -39,7 +39,48 @@ void run(const F& f, int64_t n, const double* y0, const double* t, int nt, doubl
     }
 }
 
+template <class F>
+void run54(const F& f, int64_t n, const double* y0, const double* t, int nt, double rtol,
+           double atol, double dt_one_init, double* out, int64_t* stats) {
+    Dopr54Job J;
+    J.times = t;
+    J.nt = nt;
+    J.max_attempts = 1u << 22;
+    J.dt = t[1] - t[0];
+    J.rtol = rtol;
+    J.atol = atol;
+    J.dt_one_init = dt_one_init;
+    dopr54_bounds(J.bound2);
+    for (int64_t i = 0; i < n; i++) {
+        HostSink sink{out, n, i};
+        Dopr54Lane L;
+        dopr54_init(L, y0 + DIM * i, t[0], J, f, sink);
+        bool done = L.ii >= J.nt - 1;
+        while (!done) done = dopr54_attempt(L, J, f, sink);
+        stats[4 * i] = L.accepted;
+        stats[4 * i + 1] = L.rejected;
+        stats[4 * i + 2] = 0;
+        stats[4 * i + 3] = L.err | L.status;
+    }
+}
+
 }  // namespace
+
+// DOPR54 with the unit galpy Kepler force (p = NULL), Kepler or Plummer.
+extern "C" int host_dopr54(const drift_potential* p, int64_t n, const double* y0, const double* t,
+                           int nt, double rtol, double atol, double dt_one_init, double* out,
+                           int64_t* stats) {
+    if (p == nullptr)
+        run54(GalpyKeplerForce<double>{}, n, y0, t, nt, rtol, atol, dt_one_init, out, stats);
+    else if (p->kind == DRIFT_POTENTIAL_KEPLER)
+        run54(KeplerForce<double>{p->amp}, n, y0, t, nt, rtol, atol, dt_one_init, out, stats);
+    else if (p->kind == DRIFT_POTENTIAL_PLUMMER)
+        run54(PlummerForce<double>{p->amp, p->b * p->b}, n, y0, t, nt, rtol, atol, dt_one_init,
+              out, stats);
+    else
+        return 1;
+    return 0;
+}
 
 // potential: NULL = the unit galpy Kepler force; otherwise a drift_potential
 // (Kepler, Plummer or Milky-Way).  Linear tolerances.

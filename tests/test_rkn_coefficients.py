@@ -115,6 +115,10 @@ def host_lib(tmp_path_factory):
     if r.returncode != 0:
         pytest.skip(f"host build of the step machine failed: {r.stderr[-400:]}")
     lib = ctypes.CDLL(str(out))
+    lib.host_dopr54.restype = ctypes.c_int
+    lib.host_dopr54.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_void_p, ctypes.c_void_p]
     lib.host_dop853.restype = ctypes.c_int
     lib.host_dop853.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
@@ -166,3 +170,83 @@ def test_host_step_machine_matches_oracle_point_masses(host_lib, pot):
     print(f"{pot}: pos {ep:.2e} vel {ev:.2e}")
     assert ep < 1e-8 and ev < 1e-8
     assert abs(int(st[:, 0].sum()) - int(rst["accepted"].sum())) <= 1e-3 * rst["accepted"].sum()
+
+
+def test_dopr54_header_is_current():
+    r = subprocess.run([sys.executable, str(CSRC / "gen_dopr54_rkn.py"), "--check"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_dopr54_second_order_form_equals_first_order_form():
+    """With random forces, the combined coefficients reproduce the reference's
+    first-order DOPR54 combinations (dopr54.rs:486-610): stage positions, the
+    update and the error estimate."""
+    from fractions import Fraction as Fr
+
+    sys.path.insert(0, str(CSRC))
+    import gen_dopr54_rkn as g
+
+    rng = np.random.default_rng(7)
+    h = 0.37
+    x0, v0 = rng.normal(size=3), rng.normal(size=3)
+    F = {i: rng.normal(size=3) for i in range(1, 8)}
+    V = {1: v0}
+    for i in range(2, 8):
+        V[i] = v0 + h * sum(float(g.a(i, j)) * F[j] for j in range(1, i))
+    coef = {name: float(v) for name, v, _ in g.terms()}
+
+    def close(x, y):
+        np.testing.assert_allclose(x, y, rtol=1e-13, atol=1e-13)
+
+    for i in range(3, 7):
+        first = x0 + h * sum(float(g.a(i, j)) * V[j] for j in range(1, i))
+        second = x0 + h * float(g.C[i]) * v0 + h * h * sum(
+            coef[f"q{i}{m}"] * F[m] for m in range(1, i - 1))
+        close(second, first)
+    x1 = x0 + h * sum(float(g.B[i]) * V[i] for i in range(1, 7))
+    close(x0 + h * v0 + h * h * sum(coef[f"bq{m}"] * F[m] for m in range(1, 6)), x1)
+    err_pos = h * sum(float(g.E[i]) * V[i] for i in range(1, 8))
+    close(h * h * sum(coef[f"eq{m}"] * F[m] for m in range(1, 7)), err_pos)
+    assert sum(g.E.values()) == Fr(0)
+
+
+@pytest.mark.parametrize("pot", ["galpy", "kepler", "plummer"])
+def test_host_dopr54_matches_oracle(host_lib, pot):
+    spec = {"galpy": None, "kepler": _native.PotentialSpec.kepler(1.3),
+            "plummer": _native.PotentialSpec.plummer(1.0, 0.1)}[pot]
+    k = workloads.kepler_disk(n=64)
+    y0 = np.ascontiguousarray(k.state0)
+    n, nt = y0.shape[0], k.times.shape[0]
+    out = np.zeros((nt, n, 6))
+    st = np.zeros((n, 4), np.int64)
+    c = spec.to_c() if spec is not None else None
+    assert host_lib.host_dopr54(ctypes.byref(c) if c is not None else None, n, y0.ctypes.data,
+                                k.times.ctypes.data, nt, k.rtol, k.atol, -9999.99,
+                                out.ctypes.data, st.ctypes.data) == 0
+    tol = math.log(k.rtol)
+    ref, rst = oracle_lib.batch(0, y0, k.times, tol, tol, spec)
+    ep, ev = _norm_err(out, ref)
+    print(f"DOPR54 {pot}: pos {ep:.2e} vel {ev:.2e} accepted {st[:, 0].sum()} vs "
+          f"{rst['accepted'].sum()}")
+    assert ep < 1e-8 and ev < 1e-8
+    assert int(st[:, 0].sum()) == int(rst["accepted"].sum())
+    assert not np.any(st[:, 3] & 0xFF)
+
+
+def test_host_dopr54_reference_kat(host_lib):
+    """tests/galpy_integrator_tests.rs:35-42: the reference's tail bits to 1e-12."""
+    import struct
+
+    tol = struct.unpack("<d", struct.pack("<Q", 0xC03BA18A998FFFA1))[0]
+    tail = [0xBFEAD9AC890CBF36, 0xBFE1689EF5F2F535, 0, 0x3FE1689EF5F2DA37, 0xBFEAD9AC890CA90A, 0]
+    expected = np.array([struct.unpack("<d", struct.pack("<Q", b))[0] for b in tail])
+    y0 = np.array([[1.0, 0.0, 0.0, 0.0, 1.0, 0.0]])
+    t = np.linspace(0.0, 10.0, 1001)
+    out = np.zeros((t.size, 1, 6))
+    st = np.zeros((1, 4), np.int64)
+    assert host_lib.host_dopr54(None, 1, y0.ctypes.data, t.ctypes.data, t.size, math.exp(tol),
+                                math.exp(tol), -9999.99, out.ctypes.data, st.ctypes.data) == 0
+    print("tail error", np.abs(out[-1, 0] - expected).max())
+    np.testing.assert_allclose(out[-1, 0], expected, rtol=0, atol=1e-12)
+
