@@ -137,8 +137,19 @@ struct MovingForce {
     const double* origin_params;
     int32_t n_origins, n_segments;
     double t_start, inv_dt;
+    // Origin 0's track row of the segment this lane last evaluated: a lane's
+    // stage times stay inside one segment for many force evaluations, so the
+    // row is fetched again only when the segment changes, and the fetch is
+    // issued before the static force, whose evaluation hides its latency.
+    mutable int64_t row_seg = -1;
+    mutable double row[ORIGIN_COEFS];
+
+    DRIFT_HD DRIFT_FI static double quintic(const double* p, double tau) {
+        // (tau^5 .. tau^0) coefficients
+        return fma(fma(fma(fma(fma(p[0], tau, p[1]), tau, p[2]), tau, p[3]), tau, p[4]), tau, p[5]);
+    }
+
     DRIFT_HD DRIFT_FI void operator()(double t, const double* q, double* a) const {
-        base(t, q, a);
         const double u = (t - t_start) * inv_dt;
         double sf = std::floor(u);
         if (!(sf >= 0.0)) sf = 0.0;  // also a NaN time: its tau stays NaN
@@ -146,22 +157,29 @@ struct MovingForce {
         if (sf > last) sf = last;
         const double tau = u - sf;
         const int64_t seg = (int64_t)sf;
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        for (int32_t i = 0; i < n_origins; ++i) {
+        if (n_origins > 0 && seg != row_seg) {
             // rows are 144 bytes from a 256-byte aligned base: 16-byte loads
+            const double* c = static_cast<const double*>(
+                __builtin_assume_aligned(tracks + seg * ORIGIN_COEFS, 16));
+#pragma unroll
+            for (int k = 0; k < ORIGIN_COEFS; k++) row[k] = c[k];
+            row_seg = seg;
+        }
+        base(t, q, a);
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (n_origins > 0) {
+            const double* pb = static_cast<const double*>(__builtin_assume_aligned(origin_params, 16));
+            plummer_acc<double>(pb[0], pb[1], q[0] - quintic(row, tau), q[1] - quintic(row + 6, tau),
+                                q[2] - quintic(row + 12, tau), ax, ay, az);
+        }
+        for (int32_t i = 1; i < n_origins; ++i) {
             const double* c = static_cast<const double*>(__builtin_assume_aligned(
                 tracks + ((int64_t)i * n_segments + seg) * ORIGIN_COEFS, 16));
-            double o[3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const double* p = c + 6 * k;  // (tau^5 .. tau^0)
-                o[k] = fma(fma(fma(fma(fma(p[0], tau, p[1]), tau, p[2]), tau, p[3]), tau, p[4]),
-                           tau, p[5]);
-            }
             const double* pb =
                 static_cast<const double*>(__builtin_assume_aligned(origin_params + 2 * i, 16));
             double fx, fy, fz;
-            plummer_acc<double>(pb[0], pb[1], q[0] - o[0], q[1] - o[1], q[2] - o[2], fx, fy, fz);
+            plummer_acc<double>(pb[0], pb[1], q[0] - quintic(c, tau), q[1] - quintic(c + 6, tau),
+                                q[2] - quintic(c + 12, tau), fx, fy, fz);
             ax += fx;
             ay += fy;
             az += fz;
