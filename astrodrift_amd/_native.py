@@ -460,12 +460,46 @@ def _ptr(a: np.ndarray) -> ctypes.c_void_p:
     return ctypes.c_void_p(a.ctypes.data)
 
 
+class BoundPotential:
+    """A PotentialSpec whose moving origins stay uploaded to the session of
+    `device` until close() (or the end of a `with` block), for repeated
+    launches without a per-launch upload."""
+
+    def __init__(self, spec: PotentialSpec, device: int = 0):
+        self.session = session(device)
+        self.spec = spec
+        self.id = 0
+        o = spec.origins
+        if o is not None:
+            ident = ctypes.c_int32(0)
+            check(self.session.lib.drift_origins_create(
+                self.session.handle, _ptr(o.tracks), _ptr(o.params), o.n_origins, o.n_segments,
+                o.t_start, o.t_end, ctypes.byref(ident)))
+            self.id = ident.value
+        self.c = spec.to_c(self.id)
+
+    def close(self) -> None:
+        if self.id:
+            self.session.lib.drift_origins_destroy(self.session.handle, self.id)
+            self.id = 0
+
+    def __enter__(self) -> "BoundPotential":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+
 @contextlib.contextmanager
-def _c_potential(s: Session, potential: PotentialSpec | None):
+def _c_potential(s: Session, potential):
     """The C descriptor of `potential` (None = NULL), with its moving origins
-    uploaded to the session for the duration of the block."""
+    uploaded to the session for the duration of the block (a BoundPotential
+    keeps its own upload)."""
     if potential is None:
         yield None
+        return
+    if isinstance(potential, BoundPotential):
+        yield potential.c
         return
     o = potential.origins
     if o is None:

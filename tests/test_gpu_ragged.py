@@ -106,3 +106,80 @@ def test_ragged_empty_and_single_jobs(gpu):
                               POTS[jobs[0]["pot"]])
     assert rel(out, ref) < 1e-8
     _native.ragged_device([], POTS)  # no jobs: a no-op
+
+
+def test_ragged_lds_and_global_mw_tables_fp64_and_mixed(gpu):
+    """Two different Milky-Way potentials in one launch: the one with more
+    orbits has its table staged in LDS, the other is read from global memory.
+    Each job against the plain kernel at the same precision (the same force
+    code): FP64 and the mixed mode (FP32 stage forces, float tables), and
+    against the FP64 oracle within each mode's error budget."""
+    torch = pytest.importorskip("torch")
+    pots = [_native.PotentialSpec.milky_way(),
+            _native.PotentialSpec.milky_way(bulge_alpha=1.0, disk_frac=0.55, halo_frac=0.4)]
+    rng = np.random.default_rng(33)
+    dev = torch.device("cuda:0")
+    for precision, tol, bar in ((_native.PRECISION_FP64, 1e-10, 1e-8),
+                                (_native.PRECISION_MIXED, 1e-7, 1e-3)):
+        jobs = []
+        for k in range(8):
+            pot = 0 if k < 5 else 1  # potential 0 has more orbits: its table goes to LDS
+            n = int(rng.integers(64, 256))
+            jobs.append(dict(n=n, pot=pot, method=k % 2, times=workloads.grid_times(0.0, 10.0, 41),
+                             y0=disk_ics(pots[pot], n, rng)))
+        keep, descs = [], []
+        lt = math.log(tol)
+        for j in jobs:
+            s0 = torch.from_numpy(j["y0"]).to(dev)
+            tt = torch.from_numpy(j["times"]).to(dev)
+            out = torch.full((41, j["n"], 6), float("nan"), dtype=torch.float64, device=dev)
+            st = torch.zeros((j["n"], 4), dtype=torch.int32, device=dev)
+            keep.append((s0, tt, out, st))
+            descs.append(_native.DeviceJob(j["method"], j["pot"], j["n"], 41, lt, lt, s0.data_ptr(),
+                                           tt.data_ptr(), out.data_ptr(), st.data_ptr()))
+        print(f"ragged two MW potentials, precision={precision}", flush=True)
+        _native.ragged_device(descs, pots, precision=precision)
+        torch.cuda.synchronize()
+        worst = worst_plain = 0.0
+        for j, (s0, tt, out, st) in zip(jobs, keep):
+            o = out.cpu().numpy()
+            assert not np.isnan(o).any()
+            ref, _ = oracle_lib.batch(j["method"], j["y0"], j["times"], lt, lt, pots[j["pot"]],
+                                      nthreads=8)
+            worst = max(worst, rel(o, ref))
+            plain = torch.empty_like(out)
+            _native.launch_device(j["method"], s0.data_ptr(), tt.data_ptr(), plain.data_ptr(),
+                                  j["n"], 41, lt, lt, pots[j["pot"]], precision=precision,
+                                  schedule=_native.SCHEDULE_PLAIN)
+            torch.cuda.synchronize()
+            worst_plain = max(worst_plain, rel(o, plain.cpu().numpy()))
+        print(f"  worst rel vs oracle {worst:.3e}, vs plain kernel {worst_plain:.3e}")
+        assert worst < bar
+        assert worst_plain < 1e-8
+
+
+def test_large_ragged_batch_matches_jobs_alone_and_plain_kernel(gpu):
+    """A batch of four jobs of 9-10 thousand orbits: each job's result is
+    bit-identical to the same job run alone (an orbit's arithmetic does not
+    depend on what else the launch carries) and agrees with the plain kernel
+    (identical accepted counts)."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(77)
+    jobs = []
+    for k, (pot, method) in enumerate([(2, 1), (0, 0), (1, 1), (2, 0)]):
+        n = int(rng.integers(9000, 10000))
+        jobs.append(dict(n=n, pot=pot, method=method, times=workloads.grid_times(0.0, 8.0, 21),
+                         tol=1e-9, y0=disk_ics(POTS[pot], n, rng)))
+    print("large ragged batch:", [(j["n"], j["pot"], j["method"]) for j in jobs], flush=True)
+    results = run_ragged(jobs, torch)
+    worst = 0.0
+    for j, (out, st) in zip(jobs, results):
+        (alone, ast), = run_ragged([j], torch)
+        np.testing.assert_array_equal(out, alone)
+        np.testing.assert_array_equal(st, ast)
+        lt = math.log(j["tol"])
+        plain, pst, _ = _native.launch_host(j["method"], j["y0"], j["times"], lt, lt, POTS[j["pot"]])
+        worst = max(worst, rel(out, plain))
+        assert np.array_equal(st[:, 0], pst["accepted"].astype(np.int32))
+    print(f"  batch == jobs alone; worst rel vs plain kernel {worst:.3e}")
+    assert worst < 1e-8
