@@ -144,6 +144,47 @@ def ragged_leg(njobs: int):
     }
 
 
+def kepler_leg(steps: int, warmup: int, stream):
+    """The reference's CPU-runnable case: 1,024 Kepler orbits (the unit galpy
+    Kepler force), DOPR54 rtol = atol = 1e-10, ts = (0, 20 pi, 101), on the
+    GPU (the faster of the plain and persistent schedules, inputs resident).
+    The cpu_baseline leg times the same case on the oracle."""
+    import numpy as np
+    import torch
+
+    from astrodrift_amd import _native, workloads
+
+    k = workloads.kepler_disk()
+    dev = torch.device("cuda:0")
+    s0 = torch.from_numpy(k.state0).to(dev)
+    tt = torch.from_numpy(k.times).to(dev)
+    out = torch.empty((k.nt, k.n, 6), dtype=torch.float64, device=dev)
+    stats = torch.zeros((k.n, 4), dtype=torch.int32, device=dev)
+    lt = math.log(k.rtol)
+    runs = {}
+    for name, sched in (("plain", _native.SCHEDULE_PLAIN),
+                        ("persistent", _native.SCHEDULE_PERSISTENT)):
+        def launch():
+            _native.launch_device(_native.METHOD_DOPR54, s0.data_ptr(), tt.data_ptr(),
+                                  out.data_ptr(), k.n, k.nt, lt, lt, None, stats.data_ptr(),
+                                  schedule=sched, stream=stream.cuda_stream)
+
+        for _ in range(warmup):
+            launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            launch()
+        torch.cuda.synchronize()
+        runs[name] = (time.perf_counter() - t0) / steps
+    acc = int(stats.cpu().numpy()[:, 0].astype(np.int64).sum())
+    best = min(runs, key=runs.get)
+    return {"workload": "kepler_disk_dopr54: 1024 Kepler orbits, DOPR54 rtol=atol=1e-10, "
+                        "ts=(0, 20 pi, 101)",
+            "schedule": best, "ms": {n_: round(v * 1e3, 3) for n_, v in runs.items()},
+            "accepted": acc, "value": acc / runs[best], "unit": UNIT}
+
+
 def moving_origin_leg(steps: int, warmup: int, n: int, stream, static_ms: float):
     """The 1M-orbit Milky-Way DOP853 workload (rtol = atol = 1e-12) with one
     moving origin added to the static force: a Plummer sphere (amp 1e-3,
@@ -300,7 +341,18 @@ def cpu_baseline(w, stride: int):
                               want_out=False)
     many = time.perf_counter() - t0
     acc = int(st1["accepted"].sum())
+    # the reference's CPU-runnable case whole: 1,024 Kepler orbits, DOPR54
+    from astrodrift_amd import workloads
+
+    k = workloads.kepler_disk()
+    lk = math.log(k.rtol)
+    t0 = time.perf_counter()
+    _, kst = oracle_lib.batch(0, k.state0, k.times, lk, lk, None, nthreads=1, want_out=False)
+    kep = time.perf_counter() - t0
     return {
+        "kepler_dopr54": {"value": int(kst["accepted"].sum()) / kep, "cores": 1,
+                          "seconds": round(kep, 3),
+                          "sample": "all 1024 orbits of kepler_disk_dopr54"},
         "value": acc / one,
         "unit": UNIT,
         "cores": 1,
@@ -437,6 +489,8 @@ def main() -> int:
     torch.cuda.empty_cache()
     if not args.no_precision:
         line["precision"] = precision_leg(args.steps, args.warmup, args.n, stream)
+    line["kepler_dopr54"] = kepler_leg(args.steps, args.warmup, stream)
+    print(f"[bench] kepler dopr54: {line['kepler_dopr54']}", file=sys.stderr, flush=True)
     if not args.no_origins:
         line["moving_origin"] = moving_origin_leg(args.steps, args.warmup, args.n, stream,
                                                   r["ms_per_step"])

@@ -216,3 +216,51 @@ def test_model_api_stream_about_a_satellite(gpu):
     # the satellite's pull is visible against the stream integrated without it
     alone, _ = oracle_lib.batch(_native.METHOD_DOP853, stream0, times, lt, lt, mw_spec, nthreads=8)
     assert np.abs(alone - stream_cpu).max() > 1e-5
+
+
+def test_model_api_chained_groups(gpu):
+    """Force sources chained through two particle groups: satellite A in the
+    Milky Way; group B in the Milky Way plus A; a stream in the Milky Way plus
+    B.  Registration order differs from integration order.  Against the same
+    chain on the oracle (tracks built with the same Hermite builder)."""
+    mw = dft.Potential.bovy()
+    a0 = np.array([[1.0, 0.0, 0.05, 0.0, 1.0, 0.05]])
+    b0 = np.array([[1.2, 0.1, 0.0, -0.08, 0.95, 0.0], [1.15, -0.1, 0.02, 0.05, 0.97, 0.0]])
+    # the stream starts 0.05 from B's first particle: an orbit that passes
+    # through a point mass would make the oracle, which like the reference has
+    # no step limit, crawl at the step floor
+    stream0 = b0[0] + np.array([0.05, 0.0, 0.0, 0.0, 0.0, 0.0]) + \
+        np.random.default_rng(9).normal(0, 0.01, (96, 6))
+    ts = (0.0, 6.0, 61)
+    tol = 1e-10
+    cfg = dft.Config(engine=dft.Engine.GPU, method=dft.Method.DOP853, ts=ts, tolerance=tol,
+                     forces="model")
+    bg = dft.background(mw)
+    sat_a = dft.particles(dft.Potential.plummer(3e-3, 0.02), a0)
+    grp_b = dft.particles(dft.Potential.plummer(1e-3, 0.01), b0)
+    stream = dft.test_particles(stream0)
+    cfg.add(stream, bg, grp_b)
+    cfg.add(grp_b, bg, sat_a)
+    cfg.add(sat_a, bg)
+    print("model API: satellite -> group -> stream", flush=True)
+    got_stream, none, got_b, got_a = cfg.run()
+    assert none is None
+    times = cfg.grid.times()
+    lt = math.log(tol)
+    mws = mw.spec()
+
+    def tracks(states, spec, params):
+        acc = np.array([[oracle_lib.force_at(spec, times[k], *states[k, i, :3])
+                         for i in range(states.shape[1])] for k in range(times.size)])
+        return Origins.hermite(states, acc, times, params)
+
+    a_cpu, _ = oracle_lib.batch(_native.METHOD_DOP853, a0, times, lt, lt, mws)
+    b_spec = mws.with_origins(tracks(a_cpu, mws, (3e-3, 0.02)))
+    b_cpu, _ = oracle_lib.batch(_native.METHOD_DOP853, b0, times, lt, lt, b_spec)
+    s_spec = mws.with_origins(tracks(b_cpu, b_spec, (1e-3, 0.01)))
+    s_cpu, _ = oracle_lib.batch(_native.METHOD_DOP853, stream0, times, lt, lt, s_spec, nthreads=8)
+    for label, got, want, bar in (("A", got_a, a_cpu, 1e-8), ("B", got_b, b_cpu, 1e-7),
+                                  ("stream", got_stream, s_cpu, 1e-6)):
+        ep, ev = rel_err(got, want)
+        print(f"  {label}: pos_rel={ep:.3e} vel_rel={ev:.3e}")
+        assert ep < bar and ev < bar
