@@ -370,7 +370,7 @@ def main() -> int:
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
               "the orbits are independent and a multi-GPU run would shard them by range with no "
-              "communication, which this build does not launch.", file=sys.stderr)
+              "communication, which bench.py does not launch.", file=sys.stderr)
         return 2
 
     import numpy as np

@@ -1,7 +1,7 @@
 // drift_oracle.hpp -- CPU restatement of the reference's galpy DOPR54 / DOP853
 // integrators and force formulas.
 //
-// TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP path is compared
+// TEST INFRASTRUCTURE ONLY.  This is the CPU restatement the HIP path is compared
 // against; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
 // load it.  The product (astrodrift_amd/) never links or calls it.
 //

@@ -1,6 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
 
-Test infrastructure: the oracle is the checker the HIP path is compared
+Test infrastructure: the oracle is what the HIP path is compared
 against; it is never called by the product.
 """
 

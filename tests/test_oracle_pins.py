@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the checker.
+"""Pins the CPU oracle before the GPU results are compared with it.
 
 Each check is chosen so a plausible restatement mistake (dropped term, wrong
 sign or index, transposed coefficient) fails it:
