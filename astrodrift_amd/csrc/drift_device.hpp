@@ -53,14 +53,14 @@ constexpr int ORIGIN_COEFS = 18;
 // t in [0, 1) inside the segment.  Each segment holds TABLE_NCOEF monomial
 // coefficients of a Chebyshev interpolant in the local variable
 // s = 2t - 1 in [-1, 1].  The LDS table covers r2 in [2^TABLE_E_MIN,
-// 2^TABLE_E_MAX) (r from 1.9e-6 to 32,768).  A second, "far" table of the
+// 2^TABLE_E_MAX) (r from 2.4e-4 to 4,096; 60 KB).  A second, "far" table of the
 // same layout covers every positive finite double r2, [2^-1074, 2^1024)
 // (subnormal r2 are scaled by 2^64 to find their key); it stays in global
 // memory and is read only when an orbit leaves the LDS range.
-constexpr int TABLE_SUBBITS = 3;
-constexpr int TABLE_E_MIN = -38;
-constexpr int TABLE_E_MAX = 30;
-constexpr int TABLE_DEG = 11;
+constexpr int TABLE_SUBBITS = 4;
+constexpr int TABLE_E_MIN = -24;
+constexpr int TABLE_E_MAX = 24;
+constexpr int TABLE_DEG = 9;
 constexpr int TABLE_NCOEF = TABLE_DEG + 1;
 constexpr int TABLE_SEGMENTS = (TABLE_E_MAX - TABLE_E_MIN) << TABLE_SUBBITS;
 constexpr int TABLE_DOUBLES = TABLE_SEGMENTS * TABLE_NCOEF;
