@@ -65,10 +65,11 @@ constexpr int TABLE_NCOEF = TABLE_DEG + 1;
 constexpr int TABLE_SEGMENTS = (TABLE_E_MAX - TABLE_E_MIN) << TABLE_SUBBITS;
 constexpr int TABLE_DOUBLES = TABLE_SEGMENTS * TABLE_NCOEF;
 // The mixed-precision mode's float copy: the same segments, each with its own
-// degree-5 interpolant (interpolation error below 4e-9, under float rounding),
-// stored in 8-float rows so a row is two 16-byte loads.
-constexpr int TABLE32_NCOEF = 6;
-constexpr int TABLE32_STRIDE = 8;
+// cubic interpolant evaluated in float (worst relative error 3.3e-7, about
+// five float ulps, beside the force's other float rounding), stored in 4-float
+// rows so a row is one 16-byte load and three FMAs.
+constexpr int TABLE32_NCOEF = 4;
+constexpr int TABLE32_STRIDE = 4;
 constexpr int TABLE32_FLOATS = TABLE_SEGMENTS * TABLE32_STRIDE;
 constexpr int FAR_E_MIN = -1074;
 constexpr int FAR_E_MAX = 1024;

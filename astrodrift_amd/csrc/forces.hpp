@@ -60,7 +60,7 @@ struct PlummerForce {
 // Milky-Way model: Miyamoto-Nagai disk in closed form plus the tabulated
 // spherical part (bulge + NFW halo), see spherical_table.cpp.  `table` points
 // at the LDS table: TABLE_DOUBLES double coefficients, or for FT = float the
-// TABLE32_FLOATS of the degree-5 float table.
+// TABLE32_FLOATS of the cubic float table.
 template <class FT>
 struct MwForce {
     const PotParams* p;  // far-table pointer (outside the LDS table's range)

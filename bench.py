@@ -259,7 +259,17 @@ def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
         for sched_name, sched, staging in (
                 ("plain", _native.SCHEDULE_PLAIN, _native.STAGING_DIRECT),
                 ("persistent", _native.SCHEDULE_PERSISTENT, _native.STAGING_DIRECT),
-                ("persistent_staged", _native.SCHEDULE_PERSISTENT, _native.STAGING_STAGED)):
+                ("persistent_staged", _native.SCHEDULE_PERSISTENT, _native.STAGING_STAGED),
+                ("tuned", _native.SCHEDULE_TUNED, _native.STAGING_AUTO)):
+            if sched == _native.SCHEDULE_TUNED:
+                # the autotuner's choice for this precision (its trials run on a
+                # prefix of the grid, outside the timed loop)
+                choice = _native.tune_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(),
+                                             w.n, w.nt, lt, lt, w.potential, precision=prec,
+                                             stream=stream.cuda_stream)
+                print(f"[bench] precision {prec_name} tuned choice: {choice}", file=sys.stderr,
+                      flush=True)
+
             def launch():
                 _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(),
                                       out.data_ptr(), w.n, w.nt, lt, lt, w.potential,
@@ -305,8 +315,8 @@ def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
             np.linalg.norm(truth[:, :3], axis=1)
         position[k] = {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)),
                        "max": float(err.max())}
-    best_mixed = max(runs["mixed_persistent"]["value"], runs["mixed_persistent_staged"]["value"])
-    best_fp64 = max(runs["fp64_persistent"]["value"], runs["fp64_persistent_staged"]["value"])
+    best_mixed = max(runs[f"mixed_{k}"]["value"] for k in ("persistent", "persistent_staged", "tuned"))
+    best_fp64 = max(runs[f"fp64_{k}"]["value"] for k in ("persistent", "persistent_staged", "tuned"))
     return {
         "workload": f"mw_disk_dop853: {w.n} orbits, DOP853 rtol=atol=1e-7, {w.nt} outputs over "
                     "t=0..50; mixed = FP32 stage forces, FP64 state/error norm/step control; "
