@@ -1157,6 +1157,31 @@ int drift_tune_device(drift_session* s, int32_t method, const double* d_state0,
             }
         }
     }
+    // Output samples per accepted step over the prefix (every trial takes the
+    // same steps per orbit).  The prefix understates what direct output costs
+    // over the whole grid: there the lanes' orbits still sit near the same
+    // sample, so their scattered records still fill lines together; later
+    // they drift apart and each record leaves L2 as a partial line.  Where a
+    // step emits several samples that cost dominates (1M Milky-Way orbits,
+    // DOP853: 3.1 samples per step at rtol 1e-7, staged 59 ms against 69 ms
+    // direct; 0.74 at rtol 1e-12, direct 120 ms against 119 + 19 ms staged),
+    // so from kStagedSamplesPerStep on staged output is chosen whatever the
+    // prefix timings say.
+    constexpr double kStagedSamplesPerStep = 1.5;
+    double samples_per_step = 0.0;
+    if (rc == DRIFT_OK && have) {
+        std::vector<drift_orbit_stats> stats(n);
+        if (hipMemcpyAsync(stats.data(), s->tune_stats.ptr, n * sizeof(drift_orbit_stats),
+                           hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return fail(DRIFT_ERR_DEVICE, "autotuner: reading the trial statistics failed");
+        }
+        uint64_t accepted = 0;
+        for (const drift_orbit_stats& r : stats) accepted += r.accepted;
+        samples_per_step = accepted ? (double)(n * (k - 1)) / (double)accepted : 0.0;
+    }
     // then the fastest configuration with staged output
     if (rc == DRIFT_OK && have) {
         drift_launch_options o = chosen;
@@ -1176,7 +1201,7 @@ int drift_tune_device(drift_session* s, int32_t method, const double* d_state0,
             if (rc == DRIFT_ERR_DEVICE) set_last_error("autotuner: event timing failed");
             return rc;
         }
-        if (ms < chosen_ms) {
+        if (ms < chosen_ms || samples_per_step >= kStagedSamplesPerStep) {
             chosen = o;
             chosen_ms = ms;
         } else {

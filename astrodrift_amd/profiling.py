@@ -103,14 +103,49 @@ def _pick(counters: dict, needle: str, counter: str) -> float:
     return max(vals)  # the large dispatch of that name
 
 
-def summarize(dirs: list[str]) -> int:
+def _short(name: str) -> str:
+    return name if len(name) < 90 else name[:87] + "..."
+
+
+def long_dispatches(trace: Path, min_ms: float) -> None:
+    """Per-kernel table over the dispatches of at least min_ms, then those
+    dispatches in launch order (separates full launches from short trial runs
+    of the same kernel)."""
+    rows = []
+    for r in _csv_rows(trace, "kernel_trace.csv"):
+        ms = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+        if ms >= min_ms:
+            rows.append((int(r["Start_Timestamp"]), r["Kernel_Name"], ms))
+    rows.sort()
+    total = sum(ms for _, _, ms in rows) or 1.0
+    groups: dict[str, list[float]] = {}
+    for _, name, ms in rows:
+        groups.setdefault(name, []).append(ms)
+    print(f"Dispatches of at least {min_ms:g} ms:")
+    print()
+    print("| kernel | calls | avg duration (ms) | total (ms) | share |")
+    print("|---|---|---|---|---|")
+    for name, v in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| `{_short(name)}` | {len(v)} | {sum(v) / len(v):.3f} | {sum(v):.3f} | "
+              f"{100.0 * sum(v) / total:.2f}% |")
+    print()
+    print("| # | kernel | duration (ms) |")
+    print("|---|---|---|")
+    for k, (_, name, ms) in enumerate(rows):
+        short = name.split("(")[0].replace("void ", "").replace("drift::", "")
+        print(f"| {k} | `{short}` | {ms:.3f} |")
+
+
+def summarize(dirs: list[str], min_ms: float = 0.0) -> int:
     trace = Path(dirs[0])
+    if min_ms > 0.0:
+        long_dispatches(trace, min_ms)
+        return 0
     print("| kernel | calls | avg duration (ms) | total (ms) | share |")
     print("|---|---|---|---|---|")
     stats = _csv_rows(trace, "kernel_stats.csv")
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:8]:
-        name = r["Name"] if len(r["Name"]) < 90 else r["Name"][:87] + "..."
-        print(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.3f} | "
+        print(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.3f} | "
               f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.2f}% |")
     if len(dirs) == 1:
         return 0
@@ -161,11 +196,16 @@ def main() -> int:
     p.add_argument("--summarize", nargs="+", metavar="DIR",
                    help="trace folder, then optionally the fetch and write folders and more "
                         "counter folders")
+    p.add_argument("--min-ms", type=float, default=0.0,
+                   help="with --summarize and a trace folder alone: only dispatches of at "
+                        "least this many ms, grouped and in launch order")
     args = p.parse_args()
     if args.summarize:
         if len(args.summarize) == 2:
             p.error("--summarize takes the trace folder, then the fetch and write folders")
-        return summarize(args.summarize)
+        if args.min_ms > 0.0 and len(args.summarize) != 1:
+            p.error("--min-ms takes the trace folder alone")
+        return summarize(args.summarize, args.min_ms)
     return run(args)
 
 

@@ -142,6 +142,30 @@ def test_autotuner_choice_is_valid_and_tuned_launches_match(gpu):
     assert _same(out, ref_out) and np.array_equal(st, ref_st)
 
 
+def test_autotuner_takes_staged_output_on_a_sample_dense_grid(gpu):
+    """At rtol 1e-8 over t = 0..2 with 801 outputs every DOP853 step emits
+    many samples, so the tuner must choose staged output whatever its prefix
+    timings say; TUNED launches with that choice give the plain kernel's bits."""
+    import torch
+
+    w = workloads.mw_disk(n=5000, t_end=2.0, steps=801, rtol=1e-8, atol=1e-8)
+    dev = torch.device("cuda:0")
+    s0 = torch.from_numpy(w.state0).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    ref_out, ref_st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_PLAIN, _native.ORDER_INDEX)
+    samples_per_step = w.n * (w.nt - 1) / float(ref_st[:, 0].sum())
+    print(f"samples per accepted step over the whole grid: {samples_per_step:.2f}")
+    assert samples_per_step > 3.0
+    lt = math.log(w.rtol)
+    best = _native.tune_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(), w.n, w.nt,
+                               lt, lt, w.potential)
+    print("tuned:", best)
+    assert best["staging"] == _native.STAGING_STAGED
+    assert best["ms"] > 0.0
+    out, st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_TUNED, _native.ORDER_AUTO)
+    assert _same(out, ref_out) and np.array_equal(st, ref_st)
+
+
 def test_autotuner_rejects_bad_arguments(gpu):
     with pytest.raises(_native.DriftNativeError, match="method"):
         _native.tune_device(7, 0, 0, 10, 10, -23.0, -23.0)
