@@ -421,19 +421,24 @@ def check(code: int) -> None:
 
 
 class Session:
-    """Owns a libdrift_hip session (device, stream and cached device buffers)."""
+    """Owns a libdrift_hip session (device, stream and cached device buffers).
+
+    A session may be used by one thread at a time, and ctypes releases the
+    GIL during a foreign call, so every call on the handle holds `lock`."""
 
     def __init__(self, device: int = 0):
         self.lib = load_library()
+        self.lock = threading.RLock()
         handle = ctypes.c_void_p()
         check(self.lib.drift_session_create(int(device), ctypes.byref(handle)))
         self.handle = handle
         self.device = int(device)
 
     def close(self) -> None:
-        if self.handle:
-            self.lib.drift_session_destroy(self.handle)
-            self.handle = ctypes.c_void_p()
+        with self.lock:
+            if self.handle:
+                self.lib.drift_session_destroy(self.handle)
+                self.handle = ctypes.c_void_p()
 
     def __del__(self):  # pragma: no cover - interpreter shutdown order varies
         try:
@@ -472,15 +477,18 @@ class BoundPotential:
         o = spec.origins
         if o is not None:
             ident = ctypes.c_int32(0)
-            check(self.session.lib.drift_origins_create(
-                self.session.handle, _ptr(o.tracks), _ptr(o.params), o.n_origins, o.n_segments,
-                o.t_start, o.t_end, ctypes.byref(ident)))
+            with self.session.lock:
+                code = self.session.lib.drift_origins_create(
+                    self.session.handle, _ptr(o.tracks), _ptr(o.params), o.n_origins,
+                    o.n_segments, o.t_start, o.t_end, ctypes.byref(ident))
+            check(code)
             self.id = ident.value
         self.c = spec.to_c(self.id)
 
     def close(self) -> None:
         if self.id:
-            self.session.lib.drift_origins_destroy(self.session.handle, self.id)
+            with self.session.lock:
+                self.session.lib.drift_origins_destroy(self.session.handle, self.id)
             self.id = 0
 
     def __enter__(self) -> "BoundPotential":
@@ -494,7 +502,7 @@ class BoundPotential:
 def _c_potential(s: Session, potential):
     """The C descriptor of `potential` (None = NULL), with its moving origins
     uploaded to the session for the duration of the block (a BoundPotential
-    keeps its own upload)."""
+    keeps its own upload).  Callers hold s.lock around the block."""
     if potential is None:
         yield None
         return
@@ -547,7 +555,7 @@ def launch_host(
     n, nt = y0.shape[0], t.shape[0]
     out = np.empty((nt, n, 6), dtype=np.float64)
     stats = np.zeros(n, dtype=STATS_DTYPE)
-    with _c_potential(s, potential) as pot:
+    with s.lock, _c_potential(s, potential) as pot:
         pot_ref = ctypes.byref(pot) if pot is not None else None
         if method == METHOD_DOPR54:
             code = s.lib.drift_galpy_dopr54(s.handle, _ptr(y0), _ptr(t), _ptr(out), n, nt,
@@ -589,7 +597,7 @@ def launch_device(
     opts = CLaunchOptions(precision, block_size, schedule, max_attempts, ordering, staging)
     args = (s.handle, ctypes.c_void_p(d_state0), ctypes.c_void_p(d_times), ctypes.c_void_p(d_out),
             n, nt, rtol_ln, atol_ln)
-    with _c_potential(s, potential) as pot:
+    with s.lock, _c_potential(s, potential) as pot:
         pot_ref = ctypes.byref(pot) if pot is not None else None
         if method == METHOD_DOPR54:
             code = s.lib.drift_galpy_dopr54_device(*args, dt_one, pot_ref,
@@ -610,7 +618,7 @@ def tune_device(method: int, d_state0: int, d_times: int, n: int, nt: int, rtol_
     s = session(device)
     best = CLaunchOptions(precision, 0, 0, 0, 0, 0)
     ms = ctypes.c_double(0.0)
-    with _c_potential(s, potential) as pot:
+    with s.lock, _c_potential(s, potential) as pot:
         code = s.lib.drift_tune_device(s.handle, method, ctypes.c_void_p(d_state0),
                                        ctypes.c_void_p(d_times), n, nt, rtol_ln, atol_ln, dt_one,
                                        ctypes.byref(pot) if pot is not None else None,
@@ -654,8 +662,10 @@ def ragged_device(jobs: list[DeviceJob], potentials: list, precision: int = PREC
     pots = (CPotential * len(potentials))(
         *[(p.to_c() if p is not None else PotentialSpec.galpy_kepler().to_c()) for p in potentials])
     opts = CLaunchOptions(precision, block_size, 0, max_attempts, ORDER_AUTO, 0)
-    check(s.lib.drift_ragged_device(s.handle, arr, len(jobs), pots, len(potentials),
-                                    ctypes.byref(opts), ctypes.c_void_p(stream)))
+    with s.lock:
+        code = s.lib.drift_ragged_device(s.handle, arr, len(jobs), pots, len(potentials),
+                                         ctypes.byref(opts), ctypes.c_void_p(stream))
+    check(code)
 
 
 def force_device(potential: PotentialSpec | None, d_pos: int, d_acc: int, n: int,
@@ -664,7 +674,7 @@ def force_device(potential: PotentialSpec | None, d_pos: int, d_acc: int, n: int
     """drift_force_device: accelerations at n points (d_t: n times, or 0 for
     t = 0)."""
     s = session(device)
-    with _c_potential(s, potential) as pot:
+    with s.lock, _c_potential(s, potential) as pot:
         code = s.lib.drift_force_device(s.handle, ctypes.byref(pot) if pot is not None else None,
                                         ctypes.c_void_p(d_pos), ctypes.c_void_p(d_t),
                                         ctypes.c_void_p(d_acc), n, precision,
@@ -680,7 +690,7 @@ def force_host(potential: PotentialSpec | None, pos, t=None, device: int = 0) ->
     n = p.shape[0]
     tt = None if t is None else np.ascontiguousarray(np.broadcast_to(t, (n,)), dtype=np.float64)
     acc = np.empty((n, 3))
-    with _c_potential(s, potential) as pot:
+    with s.lock, _c_potential(s, potential) as pot:
         code = s.lib.drift_force(s.handle, ctypes.byref(pot) if pot is not None else None,
                                  _ptr(p), _ptr(tt) if tt is not None else None, _ptr(acc), n)
     check(code)

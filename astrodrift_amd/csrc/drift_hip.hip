@@ -1019,7 +1019,8 @@ int drift_session_create(int device, drift_session** out) {
 int drift_session_destroy(drift_session* s) {
     if (!s) return DRIFT_OK;
     (void)hipSetDevice(s->device);
-    (void)hipStreamSynchronize(s->stream);
+    // launches on the caller's streams may still use the session's buffers
+    (void)hipDeviceSynchronize();
     for (auto& t : s->tables) {
         (void)hipFree(t.d_table64);
         (void)hipFree(t.d_table32);
@@ -1281,6 +1282,8 @@ int drift_origins_destroy(drift_session* session, int32_t origins_id) {
         OriginSet& o = session->origin_sets[k];
         if (o.id != origins_id) continue;
         DRIFT_HIP_TRY(hipSetDevice(session->device));
+        // launches already enqueued on any stream may still read the set
+        DRIFT_HIP_TRY(hipDeviceSynchronize());
         DRIFT_HIP_TRY(hipFree(o.d_tracks));
         DRIFT_HIP_TRY(hipFree(o.d_params));
         session->origin_sets.erase(session->origin_sets.begin() + (ptrdiff_t)k);

@@ -170,6 +170,8 @@ const char* drift_last_error(void);
 /* Creates a session bound to HIP device `device` with its own stream
  * (replaces the per-launch CudaContext::new(0), gpu/cuda_oxide.rs:16-18). */
 int drift_session_create(int device, drift_session** out);
+/* Waits for the device (launches on any stream may use the session's
+ * buffers), then frees everything the session holds. */
 int drift_session_destroy(drift_session* session);
 
 /* ---- host-buffer launches (GpuKernel::GalpyDopr54 / GalpyDop853 through
@@ -339,7 +341,8 @@ int drift_server_destroy(drift_server* server);
 int drift_origins_create(drift_session* session, const double* tracks, const double* params,
                          int32_t n_origins, int32_t n_segments, double t_start, double t_end,
                          int32_t* origins_id);
-/* Frees a set.  No launch that uses it may still be running. */
+/* Frees a set.  Waits for the device first, so launches already enqueued
+ * that use it (on any stream) finish; later launches must not name it. */
 int drift_origins_destroy(drift_session* session, int32_t origins_id);
 
 /* Host-only: quintic Hermite tracks of n orbits sampled at nt strictly
