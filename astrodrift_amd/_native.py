@@ -49,7 +49,7 @@ STAGING_AUTO = 0
 STAGING_DIRECT = 1
 STAGING_STAGED = 2
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 STATUS_CODE_MASK = 0xFF
 STATUS_NONFINITE = 0x100

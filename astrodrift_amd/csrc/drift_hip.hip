@@ -1037,10 +1037,7 @@ int drift_session_destroy(drift_session* s) {
     if (s->order_done) (void)hipEventDestroy(s->order_done);
     if (s->staging_done) (void)hipEventDestroy(s->staging_done);
     if (s->queues) (void)hipFree(s->queues);
-    for (auto& r : s->ragged) {
-        if (r.desc.ptr) (void)hipFree(r.desc.ptr);
-        if (r.queue) (void)hipFree(r.queue);
-    }
+    for (auto& r : s->ragged) ragged_scratch_free(r);
     (void)hipStreamDestroy(s->stream);
     delete s;
     return DRIFT_OK;

@@ -370,10 +370,11 @@ __global__ void ragged_clear_u32(unsigned int* p, int64_t n) {
 // Host side
 // ---------------------------------------------------------------------------
 
-int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
-                  const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
-                  const std::vector<const void*>& t32, const drift_launch_options& opt,
-                  hipStream_t stream, RaggedScratch& scratch) {
+int ragged_prepare(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
+                   const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
+                   const std::vector<const void*>& t32, const drift_launch_options& opt,
+                   RaggedScratch& scratch, RaggedLaunch& launch) {
+    launch = RaggedLaunch{};
     if (njobs == 0) return DRIFT_OK;
     const int64_t items = host_jobs[njobs - 1].item_begin + host_jobs[njobs - 1].n;
     if (items == 0) return DRIFT_OK;
@@ -384,17 +385,26 @@ int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, i
     const size_t total = jobs_bytes + pots_bytes + 2 * tab_bytes;
     int rc = ensure_buffer(scratch.desc, total);
     if (rc != DRIFT_OK) return rc;
-    scratch.staging.resize(total);
-    char* h = scratch.staging.data();
+    // the pinned staging may still feed the last eager upload
+    if (scratch.upload_done) DRIFT_HIP_TRY(hipEventSynchronize(scratch.upload_done));
+    if (scratch.pinned_bytes < total) {
+        if (scratch.pinned) DRIFT_HIP_TRY(hipHostFree(scratch.pinned));
+        scratch.pinned = nullptr;
+        scratch.pinned_bytes = 0;
+        DRIFT_HIP_TRY(hipHostMalloc((void**)&scratch.pinned, total, hipHostMallocDefault));
+        scratch.pinned_bytes = total;
+    }
+    if (!scratch.upload_done)
+        DRIFT_HIP_TRY(hipEventCreateWithFlags(&scratch.upload_done, hipEventDisableTiming));
+    char* h = scratch.pinned;
     std::memcpy(h, host_jobs, jobs_bytes);
     std::memcpy(h + jobs_bytes, pots.data(), pots_bytes);
     std::memcpy(h + jobs_bytes + pots_bytes, t64.data(), tab_bytes);
     std::memcpy(h + jobs_bytes + pots_bytes + tab_bytes, t32.data(), tab_bytes);
-    DRIFT_HIP_TRY(hipMemcpyAsync(scratch.desc.ptr, h, total, hipMemcpyHostToDevice, stream));
     if (!scratch.queue) DRIFT_HIP_TRY(hipMalloc(&scratch.queue, sizeof(unsigned long long)));
 
     char* d = static_cast<char*>(scratch.desc.ptr);
-    RaggedArgs a{};
+    RaggedArgs& a = launch.args;
     a.jobs = reinterpret_cast<const RaggedJob*>(d);
     a.njobs = njobs;
     a.items = items;
@@ -418,23 +428,56 @@ int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, i
             }
         if (a.lds_pot >= 0) a.lds_src = mixed ? t32[a.lds_pot] : t64[a.lds_pot];
     }
-
-    using K = void (*)(RaggedArgs);
-    K k;
     if (method == DRIFT_METHOD_DOP853)
-        k = mixed ? ragged_dop853_kernel<float> : ragged_dop853_kernel<double>;
+        launch.kernel = mixed ? ragged_dop853_kernel<float> : ragged_dop853_kernel<double>;
     else
-        k = mixed ? ragged_dopr54_kernel<float> : ragged_dopr54_kernel<double>;
-    const int block = opt.block_size ? opt.block_size : 256;
+        launch.kernel = mixed ? ragged_dopr54_kernel<float> : ragged_dopr54_kernel<double>;
+    launch.block = opt.block_size ? opt.block_size : 256;
     int per_cu = 0;
-    DRIFT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0));
-    int64_t blocks = (int64_t)std::max(per_cu, 1) * std::max(s->cu_count, 1);
-    blocks = std::min<int64_t>(blocks, (items + block - 1) / block);
-    hipLaunchKernelGGL(ragged_reset_queue, dim3(1), dim3(1), 0, stream, a.queue);
+    DRIFT_HIP_TRY(
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, launch.kernel, launch.block, 0));
+    const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(s->cu_count, 1);
+    launch.blocks = std::min<int64_t>(resident, (items + launch.block - 1) / launch.block);
+    launch.bytes = total;
+    return DRIFT_OK;
+}
+
+int ragged_enqueue(const RaggedLaunch& launch, RaggedScratch& scratch, hipStream_t stream,
+                   bool capturing) {
+    if (!launch.kernel) return DRIFT_OK;  // nothing to integrate
+    DRIFT_HIP_TRY(hipMemcpyAsync(scratch.desc.ptr, scratch.pinned, launch.bytes,
+                                 hipMemcpyHostToDevice, stream));
+    // under capture the caller synchronises before it rewrites the staging
+    if (!capturing) DRIFT_HIP_TRY(hipEventRecord(scratch.upload_done, stream));
+    hipLaunchKernelGGL(ragged_reset_queue, dim3(1), dim3(1), 0, stream, launch.args.queue);
     DRIFT_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, stream, a);
+    hipLaunchKernelGGL(launch.kernel, dim3((unsigned)launch.blocks), dim3(launch.block), 0,
+                       stream, launch.args);
     DRIFT_HIP_TRY(hipGetLastError());
     return DRIFT_OK;
+}
+
+int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
+                  const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
+                  const std::vector<const void*>& t32, const drift_launch_options& opt,
+                  hipStream_t stream, RaggedScratch& scratch) {
+    RaggedLaunch launch;
+    int rc = ragged_prepare(s, host_jobs, njobs, method, pots, t64, t32, opt, scratch, launch);
+    if (rc != DRIFT_OK) return rc;
+    return ragged_enqueue(launch, scratch, stream, false);
+}
+
+void ragged_scratch_free(RaggedScratch& scratch) {
+    if (scratch.desc.ptr) (void)hipFree(scratch.desc.ptr);
+    scratch.desc.ptr = nullptr;
+    scratch.desc.bytes = 0;
+    if (scratch.pinned) (void)hipHostFree(scratch.pinned);
+    scratch.pinned = nullptr;
+    scratch.pinned_bytes = 0;
+    if (scratch.upload_done) (void)hipEventDestroy(scratch.upload_done);
+    scratch.upload_done = nullptr;
+    if (scratch.queue) (void)hipFree(scratch.queue);
+    scratch.queue = nullptr;
 }
 
 int ragged_clear(double* resident, int64_t n, unsigned int* done_count, hipStream_t stream) {

@@ -52,16 +52,41 @@ struct RaggedArgs {
     const void* lds_src;   // that table in global memory (FP64 or the mixed mode's FP32)
 };
 
+// Per-method launch scratch: the device copy of the descriptors (jobs | pots |
+// table pointers), their pinned host staging (uploaded with an asynchronous
+// copy; upload_done marks the end of the last eager upload) and the work-queue
+// counter.
 struct RaggedScratch {
     DeviceBuffer desc;
-    std::vector<char> staging;
+    char* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    hipEvent_t upload_done = nullptr;
     unsigned long long* queue = nullptr;
 };
 
+// One ragged launch, prepared on the host: descriptors written to the pinned
+// staging, device buffers allocated, grid chosen.  ragged_enqueue then only
+// records stream work (an upload, the queue reset, the kernel), so it may run
+// under stream capture.
+struct RaggedLaunch {
+    RaggedArgs args{};
+    void (*kernel)(RaggedArgs) = nullptr;
+    int64_t blocks = 0;
+    int block = 0;
+    size_t bytes = 0;  // descriptor bytes to upload
+};
+
+int ragged_prepare(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
+                   const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
+                   const std::vector<const void*>& t32, const drift_launch_options& opt,
+                   RaggedScratch& scratch, RaggedLaunch& launch);
+int ragged_enqueue(const RaggedLaunch& launch, RaggedScratch& scratch, hipStream_t stream,
+                   bool capturing);
 int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
                   const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
                   const std::vector<const void*>& t32, const drift_launch_options& opt,
                   hipStream_t stream, RaggedScratch& scratch);
+void ragged_scratch_free(RaggedScratch& scratch);
 int ragged_clear(double* resident, int64_t n, unsigned int* done_count, hipStream_t stream);
 int resolve_pots(drift_session* s, const drift_potential* potentials, size_t npots,
                  std::vector<PotParams>& pots, std::vector<const void*>& t64,

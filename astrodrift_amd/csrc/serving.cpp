@@ -23,6 +23,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -89,6 +90,20 @@ struct drift_server {
     drift::Clock::time_point t0;
     drift::RaggedScratch scratch[2];
     int32_t chunk_now = 8;  // samples per chunk (adaptive when opt.chunk_samples == 0)
+    // A chunk's stream work (per method: descriptor upload, queue reset,
+    // ragged kernel; then the finished-orbit counters back to pinned memory),
+    // captured into a hipGraph and replayed while the chunk's shape holds:
+    // the kernels' arguments, grids and the buffers the copies use.  What
+    // changes from chunk to chunk (each job's stop sample) lives in the
+    // uploaded descriptors.  A new admitted set, a finished request or a
+    // reallocated buffer changes the shape and the next chunk is captured anew.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
+    bool graphs_ok = true;  // false once capture failed: chunks are then launched eagerly
+    unsigned int* done_pinned = nullptr;  // finished-orbit count of each active request
+    size_t done_capacity = 0;
+    std::atomic<uint64_t> graph_launches{0}, graph_captures{0}, eager_chunks{0};
 };
 
 namespace drift {
@@ -190,6 +205,7 @@ int run_chunk(drift_server* S, std::vector<std::shared_ptr<Request>>& active, hi
     lo.block_size = S->opt.block_size;
     lo.max_attempts = S->opt.max_attempts;
     const int32_t chunk = S->opt.chunk_samples > 0 ? S->opt.chunk_samples : S->chunk_now;
+    RaggedLaunch launches[2];
     for (int method : {DRIFT_METHOD_DOP853, DRIFT_METHOD_DOPR54}) {
         std::vector<size_t> idx;
         for (size_t k = 0; k < active.size(); ++k)
@@ -221,14 +237,83 @@ int run_chunk(drift_server* S, std::vector<std::shared_ptr<Request>>& active, hi
             begin += j.n;
             jobs.push_back(j);
         }
-        rc = ragged_launch(S->session, jobs.data(), (int32_t)jobs.size(), method, pots, t64, t32,
-                           lo, st, S->scratch[method]);
+        rc = ragged_prepare(S->session, jobs.data(), (int32_t)jobs.size(), method, pots, t64,
+                            t32, lo, S->scratch[method], launches[method]);
         if (rc != DRIFT_OK) return rc;
     }
-    for (auto& r : active)
-        DRIFT_HIP_TRY(hipMemcpyAsync(&r->host_done, r->d_done, sizeof(unsigned int),
-                                     hipMemcpyDeviceToHost, st));
+    if (S->done_capacity < active.size()) {
+        if (S->done_pinned) DRIFT_HIP_TRY(hipHostFree(S->done_pinned));
+        S->done_pinned = nullptr;
+        S->done_capacity = 0;
+        const size_t cap = std::max<size_t>(64, 2 * active.size());
+        DRIFT_HIP_TRY(hipHostMalloc((void**)&S->done_pinned, cap * sizeof(unsigned int),
+                                    hipHostMallocDefault));
+        S->done_capacity = cap;
+    }
+    // the chunk's shape: what the captured work bakes in
+    std::vector<uint64_t> key;
+    for (int method : {DRIFT_METHOD_DOP853, DRIFT_METHOD_DOPR54}) {
+        const RaggedLaunch& L = launches[method];
+        const RaggedArgs& a = L.args;
+        const RaggedScratch& sc = S->scratch[method];
+        for (uint64_t v : {(uint64_t)(uintptr_t)L.kernel, (uint64_t)L.blocks, (uint64_t)L.block,
+                           (uint64_t)L.bytes, (uint64_t)(uintptr_t)sc.desc.ptr,
+                           (uint64_t)(uintptr_t)sc.pinned, (uint64_t)(uintptr_t)a.jobs,
+                           (uint64_t)a.njobs, (uint64_t)a.max_attempts, (uint64_t)a.items,
+                           (uint64_t)(uintptr_t)a.pots, (uint64_t)(uintptr_t)a.tables64,
+                           (uint64_t)(uintptr_t)a.tables32, (uint64_t)(uintptr_t)a.queue,
+                           (uint64_t)(int64_t)a.lds_pot, (uint64_t)(uintptr_t)a.lds_src})
+            key.push_back(v);
+    }
+    key.push_back((uint64_t)(uintptr_t)S->done_pinned);
+    for (auto& r : active) key.push_back((uint64_t)(uintptr_t)r->d_done);
+    auto enqueue = [&](bool capturing) -> int {
+        for (int method : {DRIFT_METHOD_DOP853, DRIFT_METHOD_DOPR54}) {
+            const int erc = ragged_enqueue(launches[method], S->scratch[method], st, capturing);
+            if (erc != DRIFT_OK) return erc;
+        }
+        for (size_t k = 0; k < active.size(); ++k)
+            DRIFT_HIP_TRY(hipMemcpyAsync(&S->done_pinned[k], active[k]->d_done,
+                                         sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        return DRIFT_OK;
+    };
+    if (S->graphs_ok && (!S->graph_exec || key != S->graph_key)) {
+        if (S->graph_exec) (void)hipGraphExecDestroy(S->graph_exec);
+        if (S->graph) (void)hipGraphDestroy(S->graph);
+        S->graph_exec = nullptr;
+        S->graph = nullptr;
+        S->graph_key.clear();
+        // thread-local mode: the finisher thread keeps copying and freeing
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            S->graphs_ok = false;
+        } else {
+            const int crc = enqueue(true);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            hipGraphExec_t exec = nullptr;
+            if (crc == DRIFT_OK && ec == hipSuccess && g &&
+                hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess) {
+                S->graph = g;
+                S->graph_exec = exec;
+                S->graph_key = key;
+                S->graph_captures++;
+            } else {
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                S->graphs_ok = false;
+            }
+        }
+    }
+    if (S->graphs_ok && S->graph_exec) {
+        DRIFT_HIP_TRY(hipGraphLaunch(S->graph_exec, st));
+        S->graph_launches++;
+    } else {
+        if ((rc = enqueue(false)) != DRIFT_OK) return rc;
+        S->eager_chunks++;
+    }
+    // also orders the next chunk's rewrite of the pinned descriptors and counters
     DRIFT_HIP_TRY(hipStreamSynchronize(st));
+    for (size_t k = 0; k < active.size(); ++k) active[k]->host_done = S->done_pinned[k];
     std::lock_guard<std::mutex> lk(S->mu);
     for (auto& r : active) r->info.chunks++;
     return DRIFT_OK;
@@ -405,6 +490,15 @@ int drift_server_query(drift_server* S, uint64_t ticket, drift_request_info* inf
     return DRIFT_OK;
 }
 
+int drift_server_stats(drift_server* S, uint64_t* graph_launches, uint64_t* graph_captures,
+                       uint64_t* eager_chunks) {
+    if (!S) return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL server");
+    if (graph_launches) *graph_launches = S->graph_launches.load();
+    if (graph_captures) *graph_captures = S->graph_captures.load();
+    if (eager_chunks) *eager_chunks = S->eager_chunks.load();
+    return DRIFT_OK;
+}
+
 int drift_server_wait(drift_server* S, uint64_t ticket, double timeout_s,
                       drift_request_info* info) {
     if (!S) return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL server");
@@ -443,10 +537,12 @@ int drift_server_destroy(drift_server* S) {
     S->cv_finish.notify_all();
     if (S->finisher.joinable()) S->finisher.join();
     if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
-    for (auto& sc : S->scratch) {
-        if (sc.desc.ptr) (void)hipFree(sc.desc.ptr);
-        if (sc.queue) (void)hipFree(sc.queue);
-    }
+    (void)hipSetDevice(S->session->device);
+    (void)hipStreamSynchronize(S->session->stream);
+    if (S->graph_exec) (void)hipGraphExecDestroy(S->graph_exec);
+    if (S->graph) (void)hipGraphDestroy(S->graph);
+    if (S->done_pinned) (void)hipHostFree(S->done_pinned);
+    for (auto& sc : S->scratch) ragged_scratch_free(sc);
     drift_session_destroy(S->session);
     delete S;
     return DRIFT_OK;

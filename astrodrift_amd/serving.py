@@ -73,6 +73,9 @@ def _declare(lib) -> None:
     lib.drift_server_query.argtypes = [c.c_void_p, c.c_uint64, c.POINTER(CRequestInfo)]
     lib.drift_server_wait.restype = c.c_int
     lib.drift_server_wait.argtypes = [c.c_void_p, c.c_uint64, c.c_double, c.POINTER(CRequestInfo)]
+    lib.drift_server_stats.restype = c.c_int
+    lib.drift_server_stats.argtypes = [c.c_void_p, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64),
+                                       c.POINTER(c.c_uint64)]
     lib.drift_server_destroy.restype = c.c_int
     lib.drift_server_destroy.argtypes = [c.c_void_p]
     lib._serving_declared = True
@@ -136,6 +139,14 @@ class Server:
         info = CRequestInfo()
         _native.check(self.lib.drift_server_query(self.handle, request.ticket, ctypes.byref(info)))
         return info
+
+    def stats(self) -> dict:
+        """Chunk launches so far: replays of captured hipGraphs (graph_launches),
+        captures (one per chunk shape) and chunks launched without a graph."""
+        v = [ctypes.c_uint64() for _ in range(3)]
+        _native.check(self.lib.drift_server_stats(self.handle, *[ctypes.byref(x) for x in v]))
+        return {"graph_launches": v[0].value, "graph_captures": v[1].value,
+                "eager_chunks": v[2].value}
 
     def close(self) -> None:
         if self.handle:

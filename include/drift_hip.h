@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define DRIFT_ABI_VERSION 4
+#define DRIFT_ABI_VERSION 5
 
 /* Status codes.  They map onto the reference's error taxonomy
  * (DispatchError, src/dispatch/mod.rs:36-79; IntegrationError,
@@ -321,6 +321,16 @@ int drift_server_query(drift_server* server, uint64_t ticket, drift_request_info
  * forever).  Returns the request's drift_status, or DRIFT_ERR_TIMEOUT. */
 int drift_server_wait(drift_server* server, uint64_t ticket, double timeout_s,
                       drift_request_info* info);
+/* Chunk launches so far.  A chunk's stream work (per method: descriptor
+ * upload, queue reset, ragged kernel; then the finished-orbit counters back to
+ * host memory) is captured into a hipGraph once per chunk shape (the admitted
+ * set, the kernels' grids and buffers) and replayed while the shape holds;
+ * graph_launches counts every launch of a captured graph, graph_captures the
+ * captures, eager_chunks the chunks launched without a graph (only after a
+ * capture failed).  Any pointer may be NULL.  No counterpart in the
+ * reference, which has no serving loop. */
+int drift_server_stats(drift_server* server, uint64_t* graph_launches, uint64_t* graph_captures,
+                       uint64_t* eager_chunks);
 /* Finishes every submitted request, stops the loop and frees the server. */
 int drift_server_destroy(drift_server* server);
 

@@ -24,7 +24,8 @@ def test_header_declares_the_expected_entry_points():
         "drift_session_destroy", "drift_galpy_dopr54", "drift_galpy_dop853",
         "drift_galpy_dopr54_device", "drift_galpy_dop853_device", "drift_force_device",
         "drift_mw_table_check", "drift_origins_create", "drift_origins_destroy",
-        "drift_origin_tracks_hermite",
+        "drift_origin_tracks_hermite", "drift_server_create", "drift_server_submit",
+        "drift_server_query", "drift_server_wait", "drift_server_stats", "drift_server_destroy",
     } <= names
 
 
@@ -37,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_layouts():
     lib = _native.load_library()
-    assert lib.drift_get_abi_version() == _native.ABI_VERSION == 4
+    assert lib.drift_get_abi_version() == _native.ABI_VERSION == 5
     assert ctypes.sizeof(_native.CPotential) == 8 + 10 * 8
     assert ctypes.sizeof(_native.COrbitStats) == 16
     assert ctypes.sizeof(_native.CLaunchOptions) == 24

@@ -71,14 +71,34 @@ def test_alone_equals_mixed_and_matches_oracle(gpu):
 
 
 def test_chunking_does_not_change_results(gpu):
+    """Chunks of 1, 7 and all 60 steps of the grid give the same bits, and the
+    same bits as one eager drift_ragged_device launch of the request (the same
+    kernel, no pauses, no graph).  The server captures a chunk's work into a
+    hipGraph once and replays it for every further chunk of the same shape."""
+    import torch
+
     w = workloads.mw_disk(n=300, seed=5, t_end=6.0, steps=61, rtol=1e-11, atol=1e-11)
     outs = []
     for chunk in (1, 7, 1000):
         with serving.Server(chunk_samples=chunk) as srv:
             r = srv.submit(1, w.state0, w.times, w.rtol, w.atol, w.potential)
-            srv.wait(r, timeout_s=120)
+            info = srv.wait(r, timeout_s=120)
+            st = srv.stats()
             outs.append(r.out.copy())
+        print(f"chunk {chunk}: {info.chunks} chunks, {st}")
+        assert st["eager_chunks"] == 0
+        assert st["graph_captures"] == 1 and st["graph_launches"] == info.chunks
+        assert info.chunks == math.ceil(60 / min(chunk, 60))
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+    dev = torch.device("cuda:0")
+    s0 = torch.from_numpy(w.state0).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    out = torch.empty((w.nt, w.n, 6), dtype=torch.float64, device=dev)
+    job = _native.DeviceJob(_native.METHOD_DOP853, 0, w.n, w.nt, math.log(w.rtol),
+                            math.log(w.atol), s0.data_ptr(), tt.data_ptr(), out.data_ptr())
+    _native.ragged_device([job], [w.potential])
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), outs[0])
 
 
 def test_requests_overlap_and_budget_admission(gpu):
@@ -89,6 +109,11 @@ def test_requests_overlap_and_budget_admission(gpu):
     with serving.Server(chunk_samples=5, hbm_budget_bytes=int(2.5 * per)) as srv:
         hs = [srv.submit(1, w.state0, w.times, w.rtol, w.atol, w.potential) for _ in range(6)]
         infos = [srv.wait(h, timeout_s=120) for h in hs]
+        st = srv.stats()
+    # the admitted set changed as requests came and went: captured anew each time
+    print(st)
+    assert st["eager_chunks"] == 0 and st["graph_captures"] >= 2
+    assert st["graph_launches"] > st["graph_captures"]
     admits = sorted(i.admit_s for i in infos)
     completes = sorted(i.complete_s for i in infos)
     # at most two admitted before the first completion
