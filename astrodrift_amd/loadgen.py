@@ -61,6 +61,7 @@ def run_poisson(rate_hz: float = 10.0, duration_s: float = 8.0, n_min: int = 100
             infos.append(srv.wait(r))
             r.out = None
         wall = time.perf_counter() - start
+        chunks = srv.stats()
     lat = np.array([i.complete_s - i.submit_s for i in infos])
     wait = np.array([i.admit_s - i.submit_s for i in infos])
     first = min(i.submit_s for i in infos)
@@ -82,4 +83,5 @@ def run_poisson(rate_hz: float = 10.0, duration_s: float = 8.0, n_min: int = 100
         "chunk_target_ms": (chunk_target_ms or 10.0) if chunk_samples == 0 else None,
         "rtol": rtol,
         "precision": "mixed" if precision == _native.PRECISION_MIXED else "fp64",
+        "chunk_launches": chunks,
     }
