@@ -371,6 +371,11 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.drift_ragged_device.argtypes = [c.c_void_p, c.POINTER(CJob), c.c_size_t,
                                         c.POINTER(CPotential), c.c_size_t,
                                         c.POINTER(CLaunchOptions), c.c_void_p]
+    lib.drift_tune_ragged.restype = c.c_int
+    lib.drift_tune_ragged.argtypes = [c.c_void_p, c.POINTER(CJob), c.c_size_t,
+                                      c.POINTER(CPotential), c.c_size_t,
+                                      c.POINTER(CLaunchOptions), c.c_size_t,
+                                      c.POINTER(c.c_int32), c.c_void_p]
     lib.drift_force_device.restype = c.c_int
     lib.drift_force_device.argtypes = [c.c_void_p, c.POINTER(CPotential), c.c_void_p, c.c_void_p,
                                        c.c_void_p, c.c_size_t, c.c_int32, c.c_void_p]
@@ -666,6 +671,25 @@ def ragged_device(jobs: list[DeviceJob], potentials: list, precision: int = PREC
         code = s.lib.drift_ragged_device(s.handle, arr, len(jobs), pots, len(potentials),
                                          ctypes.byref(opts), ctypes.c_void_p(stream))
     check(code)
+
+
+def tune_ragged(jobs: list[DeviceJob], potentials: list, precision: int = PRECISION_FP64,
+                prefix_samples: int = 0, stream: int = 0, device: int = 0) -> dict:
+    """drift_tune_ragged: times block sizes 64/128/256 per method on a prefix
+    of every job's grid and stores the fastest in the session for ragged
+    launches with block_size 0."""
+    s = session(device)
+    arr = (CJob * len(jobs))(*[j.to_c() for j in jobs])
+    pots = (CPotential * len(potentials))(
+        *[(p.to_c() if p is not None else PotentialSpec.galpy_kepler().to_c()) for p in potentials])
+    opts = CLaunchOptions(precision, 0, 0, 0, ORDER_AUTO, 0)
+    best = (ctypes.c_int32 * 2)()
+    with s.lock:
+        code = s.lib.drift_tune_ragged(s.handle, arr, len(jobs), pots, len(potentials),
+                                       ctypes.byref(opts), prefix_samples, best,
+                                       ctypes.c_void_p(stream))
+    check(code)
+    return {"dopr54_block": best[METHOD_DOPR54], "dop853_block": best[METHOD_DOP853]}
 
 
 def force_device(potential: PotentialSpec | None, d_pos: int, d_acc: int, n: int,

@@ -58,6 +58,8 @@ struct drift_session {
     std::vector<bool> queue_owned;  // counters held by captured graphs
     // descriptor buffers of drift_ragged_device, one per method
     drift::RaggedScratch ragged[2];
+    // drift_tune_ragged's block size per [precision][method] (0 = untuned: 256)
+    int32_t ragged_block[2][2] = {{0, 0}, {0, 0}};
     // cost ordering: per-orbit bin keys, the slot -> orbit map and the bin
     // counters; order_done marks the end of the last launch that read them
     drift::DeviceBuffer order_keys, order_slots, order_bins;

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -616,11 +617,79 @@ extern "C" int drift_ragged_device(drift_session* session, const drift_job* jobs
             begin += r.n;
             rj.push_back(r);
         }
-        rc = ragged_launch(session, rj.data(), (int32_t)rj.size(), method, pots, t64, t32, opt, st,
+        drift_launch_options mo = opt;
+        if (!mo.block_size) mo.block_size = session->ragged_block[opt.precision][method];
+        rc = ragged_launch(session, rj.data(), (int32_t)rj.size(), method, pots, t64, t32, mo, st,
                            session->ragged[method]);
         if (rc != DRIFT_OK) return rc;
     }
     // the session's descriptor buffers are reused by the next call: wait
     DRIFT_HIP_TRY(hipStreamSynchronize(st));
+    return DRIFT_OK;
+}
+
+extern "C" int drift_tune_ragged(drift_session* session, const drift_job* jobs, size_t njobs,
+                                 const drift_potential* potentials, size_t npotentials,
+                                 const drift_launch_options* options, size_t prefix_samples,
+                                 int32_t* best_blocks, void* stream) {
+    if (!session) return fail(DRIFT_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (njobs == 0 || !jobs) return fail(DRIFT_ERR_INVALID_ARGUMENT, "no jobs to tune on");
+    int rc = validate_jobs(jobs, njobs, npotentials);
+    if (rc != DRIFT_OK) return rc;
+    drift_launch_options opt{};
+    if (options) opt = *options;
+    if (opt.precision != DRIFT_PRECISION_FP64 && opt.precision != DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "unknown precision mode");
+    // trial jobs: each job's first k outputs, written to the session's scratch
+    std::vector<drift_job> trial(jobs, jobs + njobs);
+    size_t bytes = 0;
+    for (drift_job& j : trial) {
+        size_t k = prefix_samples ? prefix_samples : std::max<size_t>(2, j.nt / 20);
+        k = std::max<size_t>(2, std::min(k, j.nt));
+        j.nt = k;
+        j.d_stats = nullptr;
+        bytes += j.n * k * DIM * sizeof(double);
+    }
+    DRIFT_HIP_TRY(hipSetDevice(session->device));
+    if ((rc = ensure_buffer(session->tune_out, std::max<size_t>(bytes, 16))) != DRIFT_OK) return rc;
+    {
+        char* base = static_cast<char*>(session->tune_out.ptr);
+        size_t off = 0;
+        for (drift_job& j : trial) {
+            j.d_state_out = reinterpret_cast<double*>(base + off);
+            off += j.n * j.nt * DIM * sizeof(double);
+        }
+    }
+    // untimed pass: resolves the tables and loads the code objects
+    drift_launch_options o = opt;
+    o.block_size = 256;
+    if ((rc = drift_ragged_device(session, trial.data(), njobs, potentials, npotentials, &o,
+                                  stream)) != DRIFT_OK)
+        return rc;
+    for (int method : {DRIFT_METHOD_DOPR54, DRIFT_METHOD_DOP853}) {
+        std::vector<drift_job> sub;
+        for (const drift_job& j : trial)
+            if (j.method == method) sub.push_back(j);
+        int32_t best = 0;
+        double best_s = INFINITY;
+        if (!sub.empty()) {
+            for (int block : {64, 128, 256}) {
+                o.block_size = block;
+                const auto t0 = std::chrono::steady_clock::now();
+                // returns once the launch has finished (it synchronises the stream)
+                if ((rc = drift_ragged_device(session, sub.data(), sub.size(), potentials,
+                                              npotentials, &o, stream)) != DRIFT_OK)
+                    return rc;
+                const double sec =
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (sec < best_s) {
+                    best_s = sec;
+                    best = block;
+                }
+            }
+        }
+        session->ragged_block[opt.precision][method] = best;
+        if (best_blocks) best_blocks[method] = best;
+    }
     return DRIFT_OK;
 }

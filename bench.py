@@ -105,6 +105,7 @@ def ragged_leg(njobs: int):
     batches.append(cur)
     acc = rej = bad = 0
     seconds = []
+    tuned = None
     for batch in batches:
         keep, descs = [], []
         for j in batch:
@@ -119,6 +120,11 @@ def ragged_leg(njobs: int):
                                            tt.data_ptr(), out.data_ptr(), st.data_ptr(),
                                            span_hint=float(j.times[-1] - j.times[0])))
         torch.cuda.synchronize()
+        if tuned is None:
+            # block size per method from a 5% prefix of the first batch's grids,
+            # outside the timed region; later launches use it
+            tuned = _native.tune_ragged(descs, list(workloads.RAGGED_POTENTIALS))
+            print(f"[bench] ragged tuned blocks: {tuned}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         _native.ragged_device(descs, list(workloads.RAGGED_POTENTIALS))
         torch.cuda.synchronize()
@@ -140,7 +146,7 @@ def ragged_leg(njobs: int):
         "dop853_jobs": sum(j.method == "DOP853" for j in jobs),
         "batches": [len(b) for b in batches], "batch_seconds": [round(x, 3) for x in seconds],
         "accepted": acc, "rejected": rej, "flagged_orbits": bad,
-        "seconds": round(wall, 3), "value": acc / wall, "unit": UNIT,
+        "seconds": round(wall, 3), "value": acc / wall, "unit": UNIT, "tuned_blocks": tuned,
     }
 
 

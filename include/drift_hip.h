@@ -261,6 +261,19 @@ int drift_ragged_device(drift_session* session, const drift_job* jobs, size_t nj
                         const drift_potential* potentials, size_t npotentials,
                         const drift_launch_options* options, void* stream);
 
+/* Autotuner for drift_ragged_device: integrates every job over its first
+ * prefix_samples outputs (0 = 5% of its grid, at least 2) with each block size
+ * in {64, 128, 256}, one method at a time, and stores the fastest per
+ * (precision, method) in the session; ragged launches of the session whose
+ * options->block_size is 0 use it (256 while untuned).  best_blocks (nullable)
+ * receives [DOPR54, DOP853] (0 = no job of that method).  Needs the prefixes'
+ * outputs as session scratch and synchronises `stream`.  No counterpart in the
+ * reference, which launches each container with a fixed 128-thread block. */
+int drift_tune_ragged(drift_session* session, const drift_job* jobs, size_t njobs,
+                      const drift_potential* potentials, size_t npotentials,
+                      const drift_launch_options* options, size_t prefix_samples,
+                      int32_t* best_blocks, void* stream);
+
 /* ---- serving loop (continuous batching) ---------------------------------- */
 
 /* A request: one job submitted from host memory.  No counterpart in the

@@ -26,6 +26,7 @@ def test_header_declares_the_expected_entry_points():
         "drift_mw_table_check", "drift_origins_create", "drift_origins_destroy",
         "drift_origin_tracks_hermite", "drift_server_create", "drift_server_submit",
         "drift_server_query", "drift_server_wait", "drift_server_stats", "drift_server_destroy",
+        "drift_ragged_device", "drift_tune_ragged", "drift_tune_device",
     } <= names
 
 

@@ -202,3 +202,42 @@ def test_large_ragged_batch_matches_jobs_alone_and_plain_kernel(gpu):
           f"{orbits} orbits with different accepted counts")
     assert worst < 1e-8
     assert differing <= 0.002 * orbits
+
+
+def test_ragged_autotuner_choice_keeps_every_bit(gpu):
+    """drift_tune_ragged picks a block size per method from {64, 128, 256};
+    launches with block_size 0 then use it, and the block size changes no
+    result: every output and statistic equals the 256-thread launch's."""
+    torch = pytest.importorskip("torch")
+    jobs = make_jobs(16, seed=33)
+    assert {j["method"] for j in jobs} == {0, 1}
+    dev = torch.device("cuda:0")
+    keep, descs = [], []
+    for j in jobs:
+        s0 = torch.from_numpy(j["y0"]).to(dev)
+        tt = torch.from_numpy(j["times"]).to(dev)
+        out = torch.full((len(j["times"]), j["n"], 6), float("nan"), dtype=torch.float64,
+                         device=dev)
+        st = torch.zeros((j["n"], 4), dtype=torch.int32, device=dev)
+        keep.append((s0, tt, out, st))
+        lt = math.log(j["tol"])
+        descs.append(_native.DeviceJob(j["method"], j["pot"], j["n"], len(j["times"]), lt, lt,
+                                       s0.data_ptr(), tt.data_ptr(), out.data_ptr(),
+                                       st.data_ptr(), span_hint=float(j["times"][-1])))
+    _native.ragged_device(descs, POTS, block_size=256)
+    torch.cuda.synchronize()
+    ref = [(o.cpu().numpy(), s.cpu().numpy()) for (_, _, o, s) in keep]
+    best = _native.tune_ragged(descs, POTS)
+    print("tuned:", best)
+    assert best["dopr54_block"] in (64, 128, 256) and best["dop853_block"] in (64, 128, 256)
+    for (_, _, o, s) in keep:
+        o.fill_(float("nan"))
+        s.zero_()
+    _native.ragged_device(descs, POTS)
+    torch.cuda.synchronize()
+    for (o, s), (ro, rs) in zip([(o.cpu().numpy(), s.cpu().numpy()) for (_, _, o, s) in keep],
+                                ref):
+        assert np.array_equal(o.view(np.uint64), ro.view(np.uint64))
+        assert np.array_equal(s, rs)
+    with pytest.raises(_native.DriftNativeError, match="no jobs"):
+        _native.tune_ragged([], POTS)
