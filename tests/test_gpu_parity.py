@@ -298,3 +298,26 @@ def test_dop853_arbitrary_times_through_the_api(gpu):
     ep, ev = rel_err(out, cpu)
     print(f"  pos_rel={ep:.3e} vel_rel={ev:.3e}")
     assert ep < REL_TOL and ev < REL_TOL
+
+
+@pytest.mark.parametrize("dt_one", [1e-3, 0.25, 2.0])
+def test_dopr54_explicit_initial_substep(gpu, dt_one):
+    """An explicit dt_one_init instead of the RK4-estimate sentinel
+    (dopr54.rs:715-717): a small one, one of about a quarter of the output
+    spacing, and one larger than the spacing, which the first sub-step clamps
+    to the interval end (:420-430).  Kepler and Milky-Way orbits against the
+    oracle with the same dt_one, with equal accepted-step totals."""
+    tol = math.log(1e-10)
+    t = workloads.grid_times(0.0, 6.0, 7)  # output spacing 1.0
+    for spec, y0 in ((None, workloads.kepler_disk(n=40, seed=17).state0),
+                     (_native.PotentialSpec.milky_way(),
+                      workloads.mw_disk(n=40, seed=18).state0)):
+        out, stats, code = _native.launch_host(_native.METHOD_DOPR54, y0, t, tol, tol, spec,
+                                               dt_one=dt_one)
+        ref, rst = oracle_lib.batch(0, y0, t, tol, tol, spec, dt_one=dt_one)
+        assert code == 0
+        ep, ev = rel_err(out, ref)
+        print(f"dt_one {dt_one} {'mw' if spec else 'kepler'}: rel {ep:.2e} / {ev:.2e}, "
+              f"accepted {int(stats['accepted'].sum())} vs {int(rst['accepted'].sum())}")
+        assert ep < REL_TOL and ev < REL_TOL
+        assert abs(int(stats["accepted"].sum()) - int(rst["accepted"].sum())) <= 2
