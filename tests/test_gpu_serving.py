@@ -129,3 +129,34 @@ def test_dopr54_error_code_reaches_the_request(gpu):
         with pytest.raises(_native.DriftNativeError):
             srv.wait(r, timeout_s=60)
         assert srv.query(r).status == _native.DRIFT_ERR_INTEGRATION
+
+
+def test_mixed_precision_server_matches_one_shot_ragged(gpu):
+    """The server in the mixed mode (FP32 stage forces, rtol 1e-7): chunked and
+    graph-replayed, the same bits as one eager mixed ragged launch of the
+    request, and within 1e-3 of the FP64 one (the mode's accuracy class)."""
+    import torch
+
+    w = workloads.mw_disk(n=500, seed=12, t_end=5.0, steps=41, rtol=1e-7, atol=1e-7)
+    with serving.Server(chunk_samples=6, precision=_native.PRECISION_MIXED) as srv:
+        r = srv.submit(1, w.state0, w.times, w.rtol, w.atol, w.potential)
+        info = srv.wait(r, timeout_s=120)
+        st = srv.stats()
+    assert info.state == serving.DONE and st["graph_launches"] == info.chunks
+    dev = torch.device("cuda:0")
+    s0 = torch.from_numpy(w.state0).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    outs = {}
+    for prec in (_native.PRECISION_MIXED, _native.PRECISION_FP64):
+        out = torch.empty((w.nt, w.n, 6), dtype=torch.float64, device=dev)
+        job = _native.DeviceJob(_native.METHOD_DOP853, 0, w.n, w.nt, math.log(w.rtol),
+                                math.log(w.atol), s0.data_ptr(), tt.data_ptr(), out.data_ptr())
+        _native.ragged_device([job], [w.potential], precision=prec)
+        torch.cuda.synchronize()
+        outs[prec] = out.cpu().numpy()
+    assert np.array_equal(r.out, outs[_native.PRECISION_MIXED])
+    ref = outs[_native.PRECISION_FP64]
+    dp = np.linalg.norm(r.out[..., :3] - ref[..., :3], axis=-1) / np.linalg.norm(ref[..., :3],
+                                                                                 axis=-1)
+    print(f"mixed server vs FP64: worst position rel diff {dp.max():.2e}")
+    assert 0.0 < dp.max() < 1e-3
