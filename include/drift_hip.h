@@ -40,6 +40,9 @@
 extern "C" {
 #endif
 
+/* 5: drift_tune_ragged, drift_server_stats.  4: moving origins, drift_force,
+ * the NONE potential.  3: drift_launch_options.staging.  2: .ordering.  A
+ * binding checks drift_get_abi_version() against the header it was built with. */
 #define DRIFT_ABI_VERSION 5
 
 /* Status codes.  They map onto the reference's error taxonomy
