@@ -59,3 +59,22 @@ def test_null_session_is_an_error_not_a_crash():
     rc = lib.drift_galpy_dop853(None, None, None, None, 1, 2, 0.0, 0.0, None, None)
     assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT
     assert "session" in _native.last_error()
+
+
+def test_null_handles_of_the_tuners_and_server_are_errors():
+    """Argument checks that run before any device call."""
+    import ctypes
+
+    lib = _native.load_library()
+    from astrodrift_amd import serving
+
+    serving._declare(lib)
+    rc = lib.drift_tune_ragged(None, None, 0, None, 0, None, 0, None, None)
+    assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT and "session" in _native.last_error()
+    rc = lib.drift_tune_device(None, 1, None, None, 10, 10, -23.0, -23.0, -9999.99, None, 0,
+                               None, None, None)
+    assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT and "session" in _native.last_error()
+    v = ctypes.c_uint64(7)
+    rc = lib.drift_server_stats(None, ctypes.byref(v), None, None)
+    assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT and v.value == 7
+    assert "server" in _native.last_error()
