@@ -848,6 +848,11 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         return fail(DRIFT_ERR_UNSUPPORTED,
                     "moving origins and the NONE potential need FP64 precision");
     a.table = opt.precision == DRIFT_PRECISION_MIXED ? t32 : t64;
+    if (capturing && opt.staging == DRIFT_STAGING_STAGED)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "staged output uses the session's staging buffer, which a later eager launch "
+                    "may reallocate, and cannot be captured into a graph; pass DRIFT_STAGING_DIRECT "
+                    "(AUTO means DIRECT under capture)");
     if (opt.schedule == DRIFT_SCHEDULE_TUNED) {
         const TunedEntry* e = find_tuned(s, method, a.pot.kind, opt.precision, n);
         opt.schedule = e ? e->options.schedule : DRIFT_SCHEDULE_PERSISTENT;
@@ -855,6 +860,12 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         opt.ordering = e ? e->options.ordering : DRIFT_ORDER_AUTO;
         if (opt.staging == DRIFT_STAGING_AUTO && e) opt.staging = e->options.staging;
         block = opt.block_size ? opt.block_size : kDefaultBlock;
+        // the tuner's session-scratch choices have capture-safe equivalents
+        // with the same results: index order and direct output
+        if (capturing) {
+            if (opt.ordering == DRIFT_ORDER_COST) opt.ordering = DRIFT_ORDER_INDEX;
+            if (opt.staging == DRIFT_STAGING_STAGED) opt.staging = DRIFT_STAGING_DIRECT;
+        }
     }
 
     if (capturing && opt.ordering == DRIFT_ORDER_COST)
@@ -900,19 +911,12 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     const size_t stage_blocks = (nt + kStageBlock - 1) / kStageBlock;
     if (staged) {
         const size_t bytes = n * stage_blocks * kStageBlock * DIM * sizeof(double);
-        if (capturing) {
-            if (s->staging.bytes < bytes)
-                return fail(DRIFT_ERR_INVALID_ARGUMENT,
-                            "stream capture: run one eager staged launch of at least this size on "
-                            "this session first (it allocates the staging buffer)");
-        } else {
-            if ((rc = ensure_buffer(s->staging, bytes)) != DRIFT_OK) return rc;
-            // the staging buffer is per session: wait for the last launch that used it
-            if (!s->staging_done)
-                DRIFT_HIP_TRY(hipEventCreateWithFlags(&s->staging_done, hipEventDisableTiming));
-            else
-                DRIFT_HIP_TRY(hipStreamWaitEvent(st, s->staging_done, 0));
-        }
+        if ((rc = ensure_buffer(s->staging, bytes)) != DRIFT_OK) return rc;
+        // the staging buffer is per session: wait for the last launch that used it
+        if (!s->staging_done)
+            DRIFT_HIP_TRY(hipEventCreateWithFlags(&s->staging_done, hipEventDisableTiming));
+        else
+            DRIFT_HIP_TRY(hipStreamWaitEvent(st, s->staging_done, 0));
         a.out = static_cast<double*>(s->staging.ptr);
         a.staged = 1;
     }
@@ -926,7 +930,7 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         hipLaunchKernelGGL(unstage_kernel, dim3((unsigned)tiles, gy), dim3(256), 0, st,
                            (const double*)s->staging.ptr, d_out, (int64_t)n, (int32_t)nt);
         DRIFT_HIP_TRY(hipGetLastError());
-        if (!capturing) DRIFT_HIP_TRY(hipEventRecord(s->staging_done, st));
+        DRIFT_HIP_TRY(hipEventRecord(s->staging_done, st));
     }
     return DRIFT_OK;
 }

@@ -583,21 +583,27 @@ extern "C" int drift_ragged_device(drift_session* session, const drift_job* jobs
                     "ragged launches write every job's output directly (staging is DIRECT)");
     if (opt.staging < DRIFT_STAGING_AUTO || opt.staging > DRIFT_STAGING_STAGED)
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "staging must be AUTO, DIRECT or STAGED");
+    // every job is checked before the first kernel is enqueued, so a refusal
+    // never leaves a launch of this call running
+    if (opt.precision == DRIFT_PRECISION_MIXED)
+        for (size_t k = 0; k < njobs; ++k)
+            if (jobs[k].n > 0 && std::exp(jobs[k].rtol) < 1e-7 * (1 - 1e-9))
+                return fail(DRIFT_ERR_UNSUPPORTED,
+                            "job " + std::to_string(k) +
+                                ": mixed precision (FP32 stage forces) requires rtol >= 1e-7");
     DRIFT_HIP_TRY(hipSetDevice(session->device));
     std::vector<PotParams> pots;
     std::vector<const void*> t64, t32;
     rc = resolve_pots(session, potentials, npotentials, pots, t64, t32);
     if (rc != DRIFT_OK) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : session->stream;
+    bool enqueued = false;
     for (int method : {DRIFT_METHOD_DOP853, DRIFT_METHOD_DOPR54}) {
         std::vector<size_t> order = ragged_order(jobs, njobs, pots, method);
         std::vector<RaggedJob> rj;
         int64_t begin = 0;
         for (size_t k : order) {
             const drift_job& j = jobs[k];
-            if (opt.precision == DRIFT_PRECISION_MIXED && std::exp(j.rtol) < 1e-7 * (1 - 1e-9))
-                return fail(DRIFT_ERR_UNSUPPORTED,
-                            "mixed precision (FP32 stage forces) requires rtol >= 1e-7");
             RaggedJob r{};
             r.state0 = j.d_state0;
             r.times = j.d_times;
@@ -621,7 +627,13 @@ extern "C" int drift_ragged_device(drift_session* session, const drift_job* jobs
         if (!mo.block_size) mo.block_size = session->ragged_block[opt.precision][method];
         rc = ragged_launch(session, rj.data(), (int32_t)rj.size(), method, pots, t64, t32, mo, st,
                            session->ragged[method]);
-        if (rc != DRIFT_OK) return rc;
+        if (rc != DRIFT_OK) {
+            // a launch of the other method may already use the outputs and the
+            // session's descriptors: let it finish before reporting the error
+            if (enqueued) (void)hipStreamSynchronize(st);
+            return rc;
+        }
+        enqueued = enqueued || !rj.empty();
     }
     // the session's descriptor buffers are reused by the next call: wait
     DRIFT_HIP_TRY(hipStreamSynchronize(st));

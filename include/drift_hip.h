@@ -135,9 +135,9 @@ typedef struct drift_launch_options {
 typedef enum drift_output_staging {
     DRIFT_STAGING_AUTO = 0,   /* the autotuner's choice for TUNED launches, else DIRECT */
     DRIFT_STAGING_DIRECT = 1,
-    DRIFT_STAGING_STAGED = 2  /* needs n * ceil(nt / 16) * 16 * 48 bytes of session scratch
-                                 (allocated by an eager launch; refused under capture
-                                 until then) */
+    DRIFT_STAGING_STAGED = 2  /* needs n * ceil(nt / 16) * 16 * 48 bytes of session scratch,
+                                 which a later launch may reallocate: refused under stream
+                                 capture */
 } drift_output_staging;
 
 /* Orbit order of a launch.  Outputs never depend on it (orbits are
@@ -211,7 +211,11 @@ int drift_galpy_dopr54_device(drift_session* session, const double* d_state0,
  * hipGraph (hipStreamBeginCapture on `stream`) after one eager launch of the
  * same session with the same potential and schedule (which builds the force
  * table and the work-queue counters; capture allocates nothing).  Under
- * capture DRIFT_ORDER_AUTO means INDEX and DRIFT_ORDER_COST is refused; each
+ * capture DRIFT_ORDER_AUTO means INDEX and DRIFT_ORDER_COST is refused,
+ * DRIFT_STAGING_AUTO means DIRECT and DRIFT_STAGING_STAGED is refused (both
+ * use session scratch that later launches reuse), and a TUNED launch takes
+ * the tuner's schedule and block size with index order and direct output
+ * (the same results); each
  * captured persistent launch keeps a work-queue counter of the session for
  * good (up to 56 per session).  A graph must not be replayed concurrently
  * with itself. */
