@@ -10,6 +10,7 @@ import contextlib
 import ctypes
 import math
 import threading
+import time
 from dataclasses import dataclass, field, replace
 from pathlib import Path
 
@@ -42,14 +43,14 @@ SCHEDULE_PLAIN = 0
 SCHEDULE_PERSISTENT = 1
 SCHEDULE_TUNED = 2  # what tune_device stored for this method / potential / precision / log2 n
 
-ORDER_AUTO = 0   # cost order for persistent launches of >= 16384 orbits
+ORDER_AUTO = 0   # cost order for launches of >= 16384 orbits (either schedule)
 ORDER_INDEX = 1
 ORDER_COST = 2
 STAGING_AUTO = 0
 STAGING_DIRECT = 1
 STAGING_STAGED = 2
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 STATUS_CODE_MASK = 0xFF
 STATUS_NONFINITE = 0x100
@@ -330,6 +331,19 @@ class CJob(ctypes.Structure):
     ]
 
 
+class CLaunchReport(ctypes.Structure):
+    _fields_ = [
+        ("setup_s", ctypes.c_double),
+        ("h2d_s", ctypes.c_double),
+        ("kernel_s", ctypes.c_double),
+        ("d2h_s", ctypes.c_double),
+        ("h2d_bytes", ctypes.c_uint64),
+        ("d2h_bytes", ctypes.c_uint64),
+        ("grid", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+    ]
+
+
 # ---------------------------------------------------------------------------
 # Library loading
 # ---------------------------------------------------------------------------
@@ -392,6 +406,12 @@ def _declare(lib: ctypes.CDLL) -> None:
                                                 c.c_size_t, c.c_void_p, _D, _D]
     lib.drift_mw_table_check.restype = c.c_int
     lib.drift_mw_table_check.argtypes = [c.POINTER(CPotential), c.c_int32, _D]
+    lib.drift_launch_shape.restype = c.c_int
+    lib.drift_launch_shape.argtypes = [c.c_void_p, c.c_int32, c.POINTER(CPotential), c.c_size_t,
+                                       c.POINTER(CLaunchOptions), c.POINTER(c.c_int32),
+                                       c.POINTER(c.c_int32)]
+    lib.drift_session_last_launch.restype = c.c_int
+    lib.drift_session_last_launch.argtypes = [c.c_void_p, c.POINTER(CLaunchReport)]
 
 
 def load_library() -> ctypes.CDLL:
@@ -432,12 +452,17 @@ class Session:
     GIL during a foreign call, so every call on the handle holds `lock`."""
 
     def __init__(self, device: int = 0):
+        started = time.perf_counter()
         self.lib = load_library()
         self.lock = threading.RLock()
         handle = ctypes.c_void_p()
         check(self.lib.drift_session_create(int(device), ctypes.byref(handle)))
         self.handle = handle
         self.device = int(device)
+        # seconds taken to load the library and create the session (device
+        # context and stream); reported once with the first launch's setup
+        self.created_s = time.perf_counter() - started
+        self.creation_reported = False
 
     def close(self) -> None:
         with self.lock:
@@ -571,6 +596,40 @@ def launch_host(
     if code != DRIFT_OK and (code != DRIFT_ERR_INTEGRATION or raise_on_integration_error):
         check(code)
     return out, stats, code
+
+
+def launch_shape(method: int, n: int, potential: PotentialSpec | None = None,
+                 precision: int = PRECISION_FP64, schedule: int | None = None,
+                 block_size: int = 0, device: int = 0) -> tuple[int, int]:
+    """drift_launch_shape: (grid, block) of the integrator kernel for n orbits;
+    schedule None is the host-buffer entry points' (persistent) shape."""
+    s = session(device)
+    if potential is None:
+        pot = None
+    elif isinstance(potential, BoundPotential):
+        pot = potential.c
+    else:
+        # only the presence of moving origins matters to the shape
+        pot = potential.to_c(1 if potential.origins is not None else 0)
+    opts = None if schedule is None else CLaunchOptions(precision, block_size, schedule, 0, 0, 0)
+    grid, block = ctypes.c_int32(0), ctypes.c_int32(0)
+    with s.lock:
+        check(s.lib.drift_launch_shape(s.handle, method,
+                                       ctypes.byref(pot) if pot is not None else None, n,
+                                       ctypes.byref(opts) if opts is not None else None,
+                                       ctypes.byref(grid), ctypes.byref(block)))
+    return grid.value, block.value
+
+
+def last_launch(device: int = 0) -> dict:
+    """drift_session_last_launch: the phases of the session's last host-buffer
+    launch (seconds; device phases from HIP events), its byte counts and the
+    integrator's launch shape."""
+    s = session(device)
+    r = CLaunchReport()
+    with s.lock:
+        check(s.lib.drift_session_last_launch(s.handle, ctypes.byref(r)))
+    return {name: getattr(r, name) for name, _ in CLaunchReport._fields_}
 
 
 def launch_device(

@@ -201,6 +201,15 @@ def _rust_display(x: float) -> str:
     return repr(float(x))
 
 
+def _rust_duration(seconds: float) -> str:
+    """A duration as the reference's records print one (Rust `{:.3?}` of a
+    Duration: the largest of s, ms, µs, ns below the value, 3 decimals)."""
+    for scale, unit in ((1.0, "s"), (1e-3, "ms"), (1e-6, "\u00b5s")):
+        if seconds >= scale:
+            return f"{seconds / scale:.3f}{unit}"
+    return f"{seconds * 1e9:.3f}ns"
+
+
 # ---------------------------------------------------------------------------
 # Potentials and containers (src/interface/recipe.rs, container.rs)
 # ---------------------------------------------------------------------------
@@ -585,30 +594,55 @@ class Config:
             stage_started = time.perf_counter()
             try:
                 potential = self._force_for(c, specs, outputs, times)
-                _log_hip.debug(
-                    "launching GALPY %s kernel: %d particles, %d output times", self.method.value,
-                    n, self.grid.steps,
+                # src/dispatch/gpu.rs:165-172, with this backend's launch shape
+                grid, block = _native.launch_shape(method, n, potential, device=self.device)
+                _log_gpu.debug(
+                    "launching GALPY %s kernel: %d particles, grid=%d, block=%d, %d output times",
+                    self.method.value, n, grid, block, len(times),
                 )
                 out, _stats, _code = _native.launch_host(
                     method, c._state, times, rtol_ln, atol_ln, potential, device=self.device
                 )
+                report = _native.last_launch(self.device)
             except _native.DriftNativeError as exc:
                 raise IntegrationError(str(exc)) from exc
+            self._log_backend_phases(report)
             outputs[c._identity] = out
             specs[c._identity] = potential
             _log_gpu.debug(
-                "stage %d: %d particles finished in %.3fs", stage, n,
-                time.perf_counter() - stage_started,
+                "stage %d: %d particles finished in %s", stage, n,
+                _rust_duration(time.perf_counter() - stage_started),
             )
         results = [outputs.get(c._identity) if c._state is not None else None
                    for c in self._containers]
         _log_gpu.info(
-            "GPU integration finished: %d stage(s), %d particle trajectories in %.3fs",
+            "GPU integration finished: %d stage(s), %d particle trajectories in %s",
             stage,
             total,
-            time.perf_counter() - started,
+            _rust_duration(time.perf_counter() - started),
         )
         return results
+
+    def _log_backend_phases(self, report: dict) -> None:
+        """The backend's per-phase records (src/dispatch/gpu/cuda_oxide.rs:15-34,
+        77-84) on drift.dispatch.gpu.hip.  The session, which holds the device
+        context and stream, is created once per process: its creation time is
+        added to the first launch's setup."""
+        if not _log_hip.isEnabledFor(logging.DEBUG):
+            return
+        sess = _native.session(self.device)
+        setup = report["setup_s"]
+        if not sess.creation_reported:
+            setup += sess.created_s
+            sess.creation_reported = True
+        _log_hip.debug("context initialized and kernel module loaded in %s",
+                       _rust_duration(setup))
+        _log_hip.debug("copied %d bytes of host data to the device in %s", report["h2d_bytes"],
+                       _rust_duration(report["h2d_s"]))
+        _log_hip.debug("integrated on the device (grid=%d, block=%d) in %s", report["grid"],
+                       report["block"], _rust_duration(report["kernel_s"]))
+        _log_hip.debug("synchronized device work and copied %d result bytes to the host in %s",
+                       report["d2h_bytes"], _rust_duration(report["d2h_s"]))
 
 
 __all__ = [

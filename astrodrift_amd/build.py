@@ -36,11 +36,18 @@ def hipcc() -> str:
     return found
 
 
-def compile_command(out: Path = LIB_PATH) -> list[str]:
+def compile_command(out: Path = LIB_PATH, fp_contract: str = "fast") -> list[str]:
+    """hipcc command line.  fp_contract="off" is a diagnostic build: the
+    compiler then fuses no multiply-add by itself (the kernels' explicit fma()
+    calls stay fused), which separates rounding differences that come from
+    contraction from those of the step control; the default contracts."""
+    if fp_contract not in ("fast", "off"):
+        raise ValueError("fp_contract must be 'fast' or 'off'")
     return [
         hipcc(),
         f"--offload-arch={ARCH}",
         "-O3",
+        f"-ffp-contract={fp_contract}",
         "-std=c++17",
         "-fPIC",
         "-shared",
@@ -57,10 +64,10 @@ def compile_command(out: Path = LIB_PATH) -> list[str]:
     ]
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, fp_contract: str = "fast") -> Path:
     """Compiles the library unconditionally and returns its path."""
     tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = compile_command(tmp)
+    cmd = compile_command(tmp, fp_contract)
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
@@ -72,4 +79,11 @@ def build(verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import argparse
+
+    ap = argparse.ArgumentParser(description="build libdrift_hip.so for gfx950")
+    ap.add_argument("--fp-contract", choices=("fast", "off"), default="fast",
+                    help="off: diagnostic build without compiler-formed multiply-adds "
+                         "(rebuild without the flag afterwards)")
+    args = ap.parse_args()
+    print(build(verbose=True, fp_contract=args.fp_contract))

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -746,8 +747,23 @@ int take_queue_slot(drift_session* s, bool capturing) {
     return -1;
 }
 
-// Orbit counts from which DRIFT_ORDER_AUTO sorts a persistent launch: below
-// it the machine is not full and the counting sort would not pay for itself.
+// Integrator grid for n orbits: one lane per orbit for the plain schedule;
+// the device's resident block count (capped by n) for the persistent one.
+int integrator_grid(const drift_session* s, PlainKernel k, int schedule, int block, size_t n,
+                    int64_t& blocks) {
+    blocks = ((int64_t)n + block - 1) / block;
+    if (schedule == DRIFT_SCHEDULE_PERSISTENT) {
+        int per_cu = 0;
+        DRIFT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0));
+        const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(s->cu_count, 1);
+        blocks = std::min(blocks, resident);
+    }
+    return DRIFT_OK;
+}
+
+// Orbit counts from which DRIFT_ORDER_AUTO sorts a launch (either schedule):
+// below it the machine is not full and the counting sort would not pay for
+// itself.
 constexpr size_t kAutoOrderMinOrbits = 16384;
 
 // Builds s->order_slots (slot -> orbit, most expensive bin first) on stream
@@ -873,7 +889,8 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
                     "cost ordering uses per-session scratch and cannot be captured into a graph; "
                     "pass DRIFT_ORDER_INDEX (AUTO means INDEX under capture)");
     PlainKernel k = kernel_for(method, a.pot.kind, opt.precision, opt.schedule);
-    int64_t blocks = ((int64_t)n + block - 1) / block;
+    int64_t blocks = 0;
+    if ((rc = integrator_grid(s, k, opt.schedule, block, n, blocks)) != DRIFT_OK) return rc;
     if (opt.schedule == 1) {
         if (!s->queues) {
             if (capturing)
@@ -889,10 +906,6 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
                         "too many captured persistent launches on this session (each graph keeps "
                         "its own work-queue counter)");
         a.queue = s->queues + slot;
-        int per_cu = 0;
-        DRIFT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0));
-        const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(s->cu_count, 1);
-        blocks = std::min(blocks, resident);
         hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, st, a.queue);
         DRIFT_HIP_TRY(hipGetLastError());
     }
@@ -900,7 +913,7 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     const bool cost_order =
         !capturing &&
         (opt.ordering == DRIFT_ORDER_COST ||
-         (opt.ordering == DRIFT_ORDER_AUTO && opt.schedule == 1 && n >= kAutoOrderMinOrbits));
+         (opt.ordering == DRIFT_ORDER_AUTO && n >= kAutoOrderMinOrbits));
     if (cost_order) {
         if (n > (size_t)UINT32_MAX)
             return fail(DRIFT_ERR_INVALID_ARGUMENT, "cost ordering supports n < 2^32");
@@ -922,6 +935,8 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     }
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, st, a);
     DRIFT_HIP_TRY(hipGetLastError());
+    s->last_grid = (int32_t)blocks;
+    s->last_block = block;
     if (cost_order) DRIFT_HIP_TRY(hipEventRecord(s->order_done, st));
     if (staged) {
         const int64_t tiles = ((int64_t)n + kUnstageOrbits - 1) / kUnstageOrbits;
@@ -943,6 +958,7 @@ int host_launch(drift_session* s, int method, const double* state0, const double
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "state0, times and state_out must be non-NULL");
     if (n < 1) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n must be >= 1");
     if (nt < 2) return fail(DRIFT_ERR_INVALID_ARGUMENT, "nt must be >= 2");
+    const auto setup_started = std::chrono::steady_clock::now();
     DRIFT_HIP_TRY(hipSetDevice(s->device));
     const size_t in_bytes = n * DIM * sizeof(double);
     const size_t out_bytes = nt * n * DIM * sizeof(double);
@@ -951,9 +967,26 @@ int host_launch(drift_session* s, int method, const double* state0, const double
     if ((rc = ensure_buffer(s->times, nt * sizeof(double))) != DRIFT_OK) return rc;
     if ((rc = ensure_buffer(s->out, out_bytes)) != DRIFT_OK) return rc;
     if ((rc = ensure_buffer(s->stats, n * sizeof(drift_orbit_stats))) != DRIFT_OK) return rc;
+    {
+        // builds (first use) or finds the force tables before anything is
+        // timed on the device
+        PotParams pp{};
+        const void *t64 = nullptr, *t32 = nullptr;
+        if ((rc = resolve_potential(s, potential, pp, &t64, &t32)) != DRIFT_OK) return rc;
+    }
+    for (hipEvent_t& e : s->phase_events)
+        if (!e) DRIFT_HIP_TRY(hipEventCreate(&e));
+    s->have_last_launch = false;
+    drift_launch_report report{};
+    report.setup_s =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - setup_started).count();
+    report.h2d_bytes = in_bytes + nt * sizeof(double);
+    report.d2h_bytes = out_bytes;
+    DRIFT_HIP_TRY(hipEventRecord(s->phase_events[0], s->stream));
     DRIFT_HIP_TRY(hipMemcpyAsync(s->state0.ptr, state0, in_bytes, hipMemcpyHostToDevice, s->stream));
     DRIFT_HIP_TRY(hipMemcpyAsync(s->times.ptr, times, nt * sizeof(double), hipMemcpyHostToDevice,
                                  s->stream));
+    DRIFT_HIP_TRY(hipEventRecord(s->phase_events[1], s->stream));
     // the fastest schedule: persistent lanes, cost-ordered from 16384 orbits
     drift_launch_options opt{};
     opt.schedule = 1;
@@ -962,6 +995,9 @@ int host_launch(drift_session* s, int method, const double* state0, const double
                            (double*)s->out.ptr, n, nt, rtol_ln, atol_ln, dt_one_init, potential,
                            (drift_orbit_stats*)s->stats.ptr, &opt, s->stream);
     if (rc != DRIFT_OK) return rc;
+    DRIFT_HIP_TRY(hipEventRecord(s->phase_events[2], s->stream));
+    report.grid = s->last_grid;
+    report.block = s->last_block;
     std::vector<drift_orbit_stats> host_stats;
     drift_orbit_stats* st = stats;
     if (!st) {
@@ -971,7 +1007,18 @@ int host_launch(drift_session* s, int method, const double* state0, const double
     DRIFT_HIP_TRY(hipMemcpyAsync(state_out, s->out.ptr, out_bytes, hipMemcpyDeviceToHost, s->stream));
     DRIFT_HIP_TRY(hipMemcpyAsync(st, s->stats.ptr, n * sizeof(drift_orbit_stats),
                                  hipMemcpyDeviceToHost, s->stream));
+    DRIFT_HIP_TRY(hipEventRecord(s->phase_events[3], s->stream));
     DRIFT_HIP_TRY(hipStreamSynchronize(s->stream));
+    {
+        float ms[3] = {0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < 3; ++k)
+            DRIFT_HIP_TRY(hipEventElapsedTime(&ms[k], s->phase_events[k], s->phase_events[k + 1]));
+        report.h2d_s = ms[0] * 1e-3;
+        report.kernel_s = ms[1] * 1e-3;
+        report.d2h_s = ms[2] * 1e-3;
+        s->last_launch = report;
+        s->have_last_launch = true;
+    }
     if (method == DRIFT_METHOD_DOPR54) {
         // integrate_kepler maps a non-zero error code to Err(code)
         // (dopr54.rs:123), reported as DispatchError::CpuIntegration.
@@ -1038,6 +1085,8 @@ int drift_session_destroy(drift_session* s) {
                             &s->order_slots, &s->order_bins, &s->tune_out, &s->tune_stats,
                             &s->staging, &s->force_io})
         if (b->ptr) (void)hipFree(b->ptr);
+    for (hipEvent_t e : s->phase_events)
+        if (e) (void)hipEventDestroy(e);
     if (s->order_done) (void)hipEventDestroy(s->order_done);
     if (s->staging_done) (void)hipEventDestroy(s->staging_done);
     if (s->queues) (void)hipFree(s->queues);
@@ -1080,6 +1129,59 @@ int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
     return launch_integrator(session, DRIFT_METHOD_DOP853, d_state0, d_times, d_state_out, n, nt,
                              rtol, atol, dp54::initial_step_sentinel, potential, d_stats, options,
                              (hipStream_t)stream);
+}
+
+int drift_launch_shape(drift_session* s, int32_t method, const drift_potential* potential,
+                       size_t n, const drift_launch_options* options, int32_t* grid,
+                       int32_t* block) {
+    if (!s || !grid || !block) return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (method != DRIFT_METHOD_DOPR54 && method != DRIFT_METHOD_DOP853)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "unknown method");
+    if (n < 1) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n must be >= 1");
+    drift_potential p{};
+    p.kind = POT_GALPY_KEPLER;
+    if (potential) p = *potential;
+    int rc = validate_potential(p);
+    if (rc != DRIFT_OK) return rc;
+    int kind = p.kind;
+    if (p.origins != 0 || p.kind == POT_NONE) kind |= POT_MOVING;
+    drift_launch_options opt{};
+    if (options)
+        opt = *options;
+    else
+        opt.schedule = DRIFT_SCHEDULE_PERSISTENT;  // what the host-buffer entry points launch
+    if (opt.precision != DRIFT_PRECISION_FP64 && opt.precision != DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "unknown precision mode");
+    if (opt.schedule < DRIFT_SCHEDULE_PLAIN || opt.schedule > DRIFT_SCHEDULE_TUNED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "schedule must be PLAIN, PERSISTENT or TUNED");
+    if ((kind & POT_MOVING) && opt.precision == DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "moving origins and the NONE potential need FP64 precision");
+    if (opt.schedule == DRIFT_SCHEDULE_TUNED) {
+        const TunedEntry* e = find_tuned(s, method, kind, opt.precision, n);
+        opt.schedule = e ? e->options.schedule : DRIFT_SCHEDULE_PERSISTENT;
+        opt.block_size = e ? e->options.block_size : 0;
+    }
+    const int b = opt.block_size ? opt.block_size : kDefaultBlock;
+    if (b < 64 || b > kDefaultBlock || b % 64 != 0)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
+    DRIFT_HIP_TRY(hipSetDevice(s->device));
+    int64_t blocks = 0;
+    rc = integrator_grid(s, kernel_for(method, kind, opt.precision, opt.schedule), opt.schedule, b,
+                         n, blocks);
+    if (rc != DRIFT_OK) return rc;
+    if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");
+    *grid = (int32_t)blocks;
+    *block = b;
+    return DRIFT_OK;
+}
+
+int drift_session_last_launch(drift_session* s, drift_launch_report* report) {
+    if (!s || !report) return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!s->have_last_launch)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "the session has made no host-buffer launch");
+    *report = s->last_launch;
+    return DRIFT_OK;
 }
 
 int drift_tune_device(drift_session* s, int32_t method, const double* d_state0,

@@ -76,6 +76,12 @@ struct drift_session {
     // moving-origin sets (drift_origins_create)
     std::vector<drift::OriginSet> origin_sets;
     int32_t next_origin_id = 1;
+    // the integrator shape of the last launch, and the phases of the last
+    // host-buffer launch (drift_session_last_launch), timed by phase_events
+    int32_t last_grid = 0, last_block = 0;
+    drift_launch_report last_launch{};
+    bool have_last_launch = false;
+    hipEvent_t phase_events[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 #define DRIFT_HIP_TRY(expr)                                                              \

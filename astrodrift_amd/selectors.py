@@ -16,24 +16,25 @@ class _StrEnum(str, Enum):
 
 
 class Engine(_StrEnum):
-    """Execution backend."""
+    """Where run() integrates: on the host (CPU) or on the MI355X (GPU).  This
+    package runs GPU configurations only; CPU ones validate but do not run."""
 
     CPU = "CPU"
     GPU = "GPU"
 
 
 class Method(_StrEnum):
-    """Adaptive integration method.
-
-    DOPR54 is Dormand-Prince 5(4); DOP853 is Dormand-Prince 8(5,3).
-    """
+    """Embedded Runge-Kutta pair with adaptive steps: DOPR54, the 5th-order
+    pair with a 4th-order error estimate, or DOP853, the 8th-order pair with
+    5th- and 3rd-order estimates and 7th-order dense output."""
 
     DOPR54 = "DOPR54"
     DOP853 = "DOP853"
 
 
 class Implementation(_StrEnum):
-    """Numerical implementation of the selected integration method."""
+    """Whose version of the method runs; GALPY (the step control of galpy's C
+    integrators) is the one with kernels."""
 
     GALPY = "GALPY"
     SCIPY = "SCIPY"

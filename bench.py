@@ -41,10 +41,10 @@ def parse_args():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=1 << 20, help="orbits (default 1,048,576)")
-    p.add_argument("--schedules", default="plain,persistent:index,persistent,tuned",
+    p.add_argument("--schedules", default="plain,plain:index,persistent:index,persistent,tuned",
                    help="comma list of schedule[:ordering] to time; schedule plain|persistent|"
                         "tuned (the autotuner's choice), ordering auto|index|cost (auto: cost "
-                        "order for persistent launches)")
+                        "order from 16,384 orbits, either schedule)")
     p.add_argument("--cpu-stride", type=int, default=512,
                    help="CPU baseline integrates every k-th orbit")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -262,11 +262,17 @@ def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
     lt = math.log(1e-7)
     runs, finals = {}, {}
     for prec_name, prec in (("fp64", _native.PRECISION_FP64), ("mixed", _native.PRECISION_MIXED)):
-        for sched_name, sched, staging in (
-                ("plain", _native.SCHEDULE_PLAIN, _native.STAGING_DIRECT),
-                ("persistent", _native.SCHEDULE_PERSISTENT, _native.STAGING_DIRECT),
-                ("persistent_staged", _native.SCHEDULE_PERSISTENT, _native.STAGING_STAGED),
-                ("tuned", _native.SCHEDULE_TUNED, _native.STAGING_AUTO)):
+        for sched_name, sched, staging, ordering in (
+                ("plain", _native.SCHEDULE_PLAIN, _native.STAGING_DIRECT, _native.ORDER_AUTO),
+                ("plain_index", _native.SCHEDULE_PLAIN, _native.STAGING_DIRECT,
+                 _native.ORDER_INDEX),
+                ("persistent", _native.SCHEDULE_PERSISTENT, _native.STAGING_DIRECT,
+                 _native.ORDER_AUTO),
+                ("persistent_staged", _native.SCHEDULE_PERSISTENT, _native.STAGING_STAGED,
+                 _native.ORDER_AUTO),
+                ("tuned", _native.SCHEDULE_TUNED, _native.STAGING_AUTO, _native.ORDER_AUTO)):
+            if prec_name == "mixed" and sched_name == "plain_index":
+                continue
             if sched == _native.SCHEDULE_TUNED:
                 # the autotuner's choice for this precision (its trials run on a
                 # prefix of the grid, outside the timed loop)
@@ -280,7 +286,8 @@ def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
                 _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(),
                                       out.data_ptr(), w.n, w.nt, lt, lt, w.potential,
                                       stats.data_ptr(), precision=prec, schedule=sched,
-                                      staging=staging, stream=stream.cuda_stream)
+                                      staging=staging, ordering=ordering,
+                                      stream=stream.cuda_stream)
 
             for _ in range(warmup):
                 launch()
@@ -328,7 +335,11 @@ def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
                     "t=0..50; mixed = FP32 stage forces, FP64 state/error norm/step control; "
                     "staged = orbit-blocked samples + tiled transpose to the time-major output",
         "runs": runs,
+        # plain = one lane per orbit in cost order (the default from 16,384
+        # orbits); plain_index = one lane per orbit in input order, the
+        # reference's launch
         "mixed_persistent_over_fp64_plain": best_mixed / runs["fp64_plain"]["value"],
+        "mixed_persistent_over_fp64_plain_index": best_mixed / runs["fp64_plain_index"]["value"],
         "mixed_persistent_over_fp64_persistent": best_mixed / best_fp64,
         "energy_rel_err_t50": energy,
         "position_rel_err_t50_vs_fp64_rtol1e-13": position,
@@ -351,7 +362,11 @@ def cpu_baseline(w, stride: int):
     _, st1 = oracle_lib.batch(1, y0, w.times, tol_r, tol_a, w.potential, nthreads=1,
                               want_out=False)
     one = time.perf_counter() - t0
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    # the host cores this process may use: its affinity set, capped by the
+    # thread share the environment grants it (OMP_NUM_THREADS, 16 if unset)
+    affinity = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    cores = max(1, min(affinity, share))
     t0 = time.perf_counter()
     _, stn = oracle_lib.batch(1, y0, w.times, tol_r, tol_a, w.potential, nthreads=cores,
                               want_out=False)
@@ -376,6 +391,7 @@ def cpu_baseline(w, stride: int):
         "sample": f"{len(idx)} orbits (every {stride}th of the {w.n}), DOP853 rtol=atol=1e-12, "
                   f"t=0..50, {acc} accepted steps, {one:.1f} s on 1 thread",
         "all_cores": {"value": int(stn["accepted"].sum()) / many, "cores": cores,
+                      "affinity_cores": affinity, "nproc": os.cpu_count(),
                       "seconds": round(many, 2)},
     }
 
@@ -412,8 +428,8 @@ def main() -> int:
     results = {}
     for name in [s.strip() for s in args.schedules.split(",") if s.strip()]:
         # "schedule" or "schedule:ordering"; the default ordering (auto) sorts
-        # persistent launches by estimated orbit cost and leaves plain ones in
-        # index order
+        # the orbits by estimated cost; "plain:index" is the reference's
+        # launch: one lane per orbit, in input order
         sched_name, _, order_name = name.partition(":")
         schedule = sched_ids[sched_name]
         ordering = order_ids[order_name or "auto"]
@@ -489,9 +505,9 @@ def main() -> int:
         },
         "roofline": {
             # compute-bound against the FP64 peak; the path has no matrix
-            # product, so the units doing the work are the FP64 vector ALUs,
-            # whose MI355X peak equals the FP64 matrix peak (78.6 TFLOP/s)
-            "bound": "mfma",
+            # product, so the units doing the work are the FP64 vector ALUs
+            # (MI355X vector FP64 peak 78.6 TFLOP/s); no MFMA is involved
+            "bound": "fp64_valu",
             "compute_unit": "fp64 vector ALU",
             "achieved": r["tflops"],
             "peak": FP64_PEAK_TFLOPS,

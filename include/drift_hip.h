@@ -40,10 +40,11 @@
 extern "C" {
 #endif
 
-/* 5: drift_tune_ragged, drift_server_stats.  4: moving origins, drift_force,
+/* 6: drift_launch_shape, drift_session_last_launch.
+ * 5: drift_tune_ragged, drift_server_stats.  4: moving origins, drift_force,
  * the NONE potential.  3: drift_launch_options.staging.  2: .ordering.  A
  * binding checks drift_get_abi_version() against the header it was built with. */
-#define DRIFT_ABI_VERSION 5
+#define DRIFT_ABI_VERSION 6
 
 /* Status codes.  They map onto the reference's error taxonomy
  * (DispatchError, src/dispatch/mod.rs:36-79; IntegrationError,
@@ -148,7 +149,7 @@ typedef enum drift_output_staging {
  * start first.  It uses per-session scratch buffers: launches of one session
  * that use it are serialised with each other, across streams, by an event. */
 typedef enum drift_ordering {
-    DRIFT_ORDER_AUTO = 0,  /* COST for the persistent schedule with n >= 16384, else INDEX */
+    DRIFT_ORDER_AUTO = 0,  /* COST with n >= 16384 orbits (either schedule), else INDEX */
     DRIFT_ORDER_INDEX = 1, /* orbit i is taken in index order */
     DRIFT_ORDER_COST = 2
 } drift_ordering;
@@ -224,6 +225,35 @@ int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
                               double rtol, double atol, const drift_potential* potential,
                               drift_orbit_stats* d_stats, const drift_launch_options* options,
                               void* stream);
+
+/* ---- launch diagnostics (the reference's debug records of
+ *      src/dispatch/gpu.rs:157-178 and gpu/cuda_oxide.rs:15-34, 77-84) -------- */
+
+/* The integrator kernel's grid (blocks) and block (threads) for a launch of
+ * n orbits: what drift_galpy_*_device would launch with these options
+ * (options NULL: the host-buffer entry points' persistent schedule).  The
+ * persistent schedule's grid is the device's resident block count, capped by
+ * n.  Allocates nothing; the reference's fixed shape is ceil(n / 128) x 128
+ * (src/dispatch/gpu.rs:23-29). */
+int drift_launch_shape(drift_session* session, int32_t method, const drift_potential* potential,
+                       size_t n, const drift_launch_options* options, int32_t* grid,
+                       int32_t* block);
+
+/* Phases of the session's last host-buffer launch (drift_galpy_dopr54 /
+ * drift_galpy_dop853), for the reference backend's per-phase records.  The
+ * device phases are HIP-event times on the session's stream. */
+typedef struct drift_launch_report {
+    double setup_s;      /* host time before the first copy: device, buffers, force tables */
+    double h2d_s;        /* state0 and times to the device */
+    double kernel_s;     /* the integration (order kernels included) */
+    double d2h_s;        /* trajectories and statistics back to the host */
+    uint64_t h2d_bytes;  /* bytes copied to the device */
+    uint64_t d2h_bytes;  /* trajectory bytes copied to the host (statistics excluded) */
+    int32_t grid, block; /* the integrator's launch shape */
+} drift_launch_report;
+
+/* DRIFT_ERR_INVALID_ARGUMENT if the session has made no host-buffer launch. */
+int drift_session_last_launch(drift_session* session, drift_launch_report* report);
 
 /* Autotuner (no counterpart in the reference).  Times every candidate
  * {PLAIN, PERSISTENT} x block {64, 128, 256} x ordering {INDEX, COST} on the

@@ -27,6 +27,7 @@ def test_header_declares_the_expected_entry_points():
         "drift_origin_tracks_hermite", "drift_server_create", "drift_server_submit",
         "drift_server_query", "drift_server_wait", "drift_server_stats", "drift_server_destroy",
         "drift_ragged_device", "drift_tune_ragged", "drift_tune_device",
+        "drift_launch_shape", "drift_session_last_launch",
     } <= names
 
 
@@ -39,8 +40,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_layouts():
     lib = _native.load_library()
-    assert lib.drift_get_abi_version() == _native.ABI_VERSION == 5
+    assert lib.drift_get_abi_version() == _native.ABI_VERSION == 6
     assert ctypes.sizeof(_native.CPotential) == 8 + 10 * 8
+    assert ctypes.sizeof(_native.CLaunchReport) == 4 * 8 + 2 * 8 + 2 * 4
     assert ctypes.sizeof(_native.COrbitStats) == 16
     assert ctypes.sizeof(_native.CLaunchOptions) == 24
     assert _native.STATS_DTYPE.itemsize == 16
@@ -74,6 +76,11 @@ def test_null_handles_of_the_tuners_and_server_are_errors():
     rc = lib.drift_tune_device(None, 1, None, None, 10, 10, -23.0, -23.0, -9999.99, None, 0,
                                None, None, None)
     assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT and "session" in _native.last_error()
+    g, b = ctypes.c_int32(3), ctypes.c_int32(4)
+    rc = lib.drift_launch_shape(None, 1, None, 10, None, ctypes.byref(g), ctypes.byref(b))
+    assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT and (g.value, b.value) == (3, 4)
+    rc = lib.drift_session_last_launch(None, ctypes.byref(_native.CLaunchReport()))
+    assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT and "NULL" in _native.last_error()
     v = ctypes.c_uint64(7)
     rc = lib.drift_server_stats(None, ctypes.byref(v), None, None)
     assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT and v.value == 7

@@ -311,3 +311,13 @@ def test_dopr54_refuses_arbitrary_times_before_launching():
             dft.background(dft.Potential.kepler(1.0)))
     with pytest.raises(NotImplementedError, match="DOP853"):
         sim.run()
+
+
+def test_durations_print_as_the_reference_records_do():
+    """Rust's `{:.3?}` of a Duration: the largest unit the value reaches."""
+    from astrodrift_amd.api import _rust_duration
+
+    assert _rust_duration(1.5) == "1.500s"
+    assert _rust_duration(0.0123456) == "12.346ms"
+    assert _rust_duration(2.5e-5) == "25.000µs"
+    assert _rust_duration(3e-8) == "30.000ns"

@@ -89,6 +89,10 @@ def test_gpu_matches_golden(gpu, path):
     dv = np.linalg.norm(got[:, 3:] - want[:, 3:], axis=1) / np.linalg.norm(want[:, 3:], axis=1)
     assert dp.max() < 1e-8 and dv.max() < 1e-8
     np.testing.assert_array_equal(got[0], want[0])
+    if path.stem == "dop853_reference":
+        # the canonical DOP853 dump: the reference's own GPU bar
+        # (tests/galpy_integrator_tests.rs:327-336)
+        assert summary["max_abs"] < 1e-12, summary
 
 
 @pytest.mark.gpu
@@ -99,4 +103,6 @@ def test_gpu_reference_kat_fixture(gpu):
                     dtype=np.uint64).view(np.float64)
     assert code == 0
     print("tail ULPs", ulp_distance(got[-1], tail))
-    assert np.max(np.abs(got[-1] - tail)) < 1e-12
+    # measured 1.7e-14: the second-order restatement and FMA contraction move
+    # the last bits (bitwise equality is the oracle's bar, test above)
+    assert np.max(np.abs(got[-1] - tail)) < 1e-13

@@ -49,14 +49,22 @@ def test_dopr54_canonical_kepler_kat(gpu):
     expected = np.array([struct.unpack("<d", struct.pack("<Q", b))[0] for b in KAT_TAIL])
     tail = out[-1, 0]
     print("GPU tail", [hex(bits(v)) for v in tail], "max |diff|", np.max(np.abs(tail - expected)))
-    np.testing.assert_allclose(tail, expected, rtol=0, atol=1e-12)
+    # measured 1.7e-14 of the reference bits: the second-order form and FMA
+    # contraction change the last bits, so the reference's bitwise bar holds
+    # for the oracle (tests/test_oracle_pins.py) and 1e-13 for the kernel
+    np.testing.assert_allclose(tail, expected, rtol=0, atol=1e-13)
     cpu, st, err = oracle_lib.orbit(0, y0[0], t, KAT_TOL, KAT_TOL)
     assert err == 0
-    np.testing.assert_allclose(out[:, 0], cpu, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(out[:, 0], cpu, rtol=0, atol=1e-13)
     assert stats["accepted"][0] == st.accepted and stats["rejected"][0] == st.rejected
 
 
-def _compare(method: int, w: workloads.Workload, idx=None, label=""):
+def _compare(method: int, w: workloads.Workload, idx=None, label="", max_count_flips=0):
+    """GPU against the oracle on the same inputs: 1e-8 relative per sample,
+    equal step totals within 0.1%, and equal per-orbit accepted and rejected
+    counts on all but `max_count_flips` orbits (a step decision within a few
+    ulps of its threshold can flip between the two roundings; such an orbit
+    still agrees to the 1e-8 bar)."""
     y0 = w.state0 if idx is None else w.state0[idx]
     rl, al = math.log(w.rtol), math.log(w.atol)
     out, stats, code = _native.launch_host(method, y0, w.times, rl, al, w.potential)
@@ -70,20 +78,25 @@ def _compare(method: int, w: workloads.Workload, idx=None, label=""):
     assert np.array_equal(out[0], y0)
     assert ep < REL_TOL and ev < REL_TOL
     assert abs(acc_g - acc_c) <= 1e-3 * acc_c
+    flips = int(np.count_nonzero((stats["accepted"] != cst["accepted"]) |
+                                 (stats["rejected"] != cst["rejected"])))
+    print(f"{label}: orbits whose accepted/rejected counts differ: {flips}")
+    assert flips <= max_count_flips
     return out, cpu, stats, cst
 
 
 def test_kepler_disk_dopr54_parity(gpu):
     """1,024 Kepler orbits, DOPR54, rtol = atol = 1e-10, ts = (0, 20 pi, 101)."""
     w = workloads.kepler_disk()
-    _compare(_native.METHOD_DOPR54, w, label="kepler_disk DOPR54")
+    _compare(_native.METHOD_DOPR54, w, label="kepler_disk DOPR54", max_count_flips=1)
 
 
 def test_mw_disk_dop853_subset_parity(gpu):
     """A strided 256-orbit subset of the 1M-orbit Milky-Way DOP853 workload."""
     w = workloads.mw_disk()
     idx = np.arange(0, w.n, w.n // 256)
-    out, cpu, _, _ = _compare(_native.METHOD_DOP853, w, idx, label="mw_disk DOP853 subset")
+    out, cpu, _, _ = _compare(_native.METHOD_DOP853, w, idx, label="mw_disk DOP853 subset",
+                              max_count_flips=3)
     # energy and Lz drift of GPU and oracle trajectories match
     spec = w.potential
 
