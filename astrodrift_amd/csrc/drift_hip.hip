@@ -749,13 +749,13 @@ int take_queue_slot(drift_session* s, bool capturing) {
 
 // Integrator grid for n orbits: one lane per orbit for the plain schedule;
 // the device's resident block count (capped by n) for the persistent one.
-int integrator_grid(const drift_session* s, PlainKernel k, int schedule, int block, size_t n,
+int integrator_grid(int cu_count, PlainKernel k, int schedule, int block, size_t n,
                     int64_t& blocks) {
     blocks = ((int64_t)n + block - 1) / block;
     if (schedule == DRIFT_SCHEDULE_PERSISTENT) {
         int per_cu = 0;
         DRIFT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0));
-        const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(s->cu_count, 1);
+        const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(cu_count, 1);
         blocks = std::min(blocks, resident);
     }
     return DRIFT_OK;
@@ -765,6 +765,9 @@ int integrator_grid(const drift_session* s, PlainKernel k, int schedule, int blo
 // below it the machine is not full and the counting sort would not pay for
 // itself.
 constexpr size_t kAutoOrderMinOrbits = 16384;
+
+int launch_cost_order(const PlainArgs& a, int precision, uint16_t* keys, uint32_t* bins,
+                      uint32_t* slots, hipStream_t st);
 
 // Builds s->order_slots (slot -> orbit, most expensive bin first) on stream
 // st.  The launch that reads it records s->order_done after its integration
@@ -780,9 +783,16 @@ int build_cost_order(drift_session* s, const PlainArgs& a, int precision, hipStr
         DRIFT_HIP_TRY(hipEventCreateWithFlags(&s->order_done, hipEventDisableTiming));
     else
         DRIFT_HIP_TRY(hipStreamWaitEvent(st, s->order_done, 0));
-    uint16_t* keys = static_cast<uint16_t*>(s->order_keys.ptr);
-    uint32_t* bins = static_cast<uint32_t*>(s->order_bins.ptr);
-    uint32_t* slots = static_cast<uint32_t*>(s->order_slots.ptr);
+    return launch_cost_order(a, precision, static_cast<uint16_t*>(s->order_keys.ptr),
+                             static_cast<uint32_t*>(s->order_bins.ptr),
+                             static_cast<uint32_t*>(s->order_slots.ptr), st);
+}
+
+// The four kernels of the cost ordering on caller-given scratch: keys (n
+// u16), bins (kOrderBins u32) and the slot -> orbit map (n u32).
+int launch_cost_order(const PlainArgs& a, int precision, uint16_t* keys, uint32_t* bins,
+                      uint32_t* slots, hipStream_t st) {
+    const size_t n = (size_t)a.n;
     const unsigned blocks = (unsigned)((n + kDefaultBlock - 1) / kDefaultBlock);
     hipLaunchKernelGGL(order_reset_kernel, dim3(1), dim3(256), 0, st, bins);
     hipLaunchKernelGGL(key_kernel_for(a.pot.kind, precision), dim3(blocks), dim3(kDefaultBlock), 0,
@@ -890,7 +900,7 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
                     "pass DRIFT_ORDER_INDEX (AUTO means INDEX under capture)");
     PlainKernel k = kernel_for(method, a.pot.kind, opt.precision, opt.schedule);
     int64_t blocks = 0;
-    if ((rc = integrator_grid(s, k, opt.schedule, block, n, blocks)) != DRIFT_OK) return rc;
+    if ((rc = integrator_grid(s->cu_count, k, opt.schedule, block, n, blocks)) != DRIFT_OK) return rc;
     if (opt.schedule == 1) {
         if (!s->queues) {
             if (capturing)
@@ -1033,6 +1043,218 @@ int host_launch(drift_session* s, int method, const double* state0, const double
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// The allocation-free DOP853 entry point.  Every buffer its kernels use beyond
+// the caller's inputs and outputs lives in `scratch`, one array per GPU for
+// the life of the process: the work-queue counter, the cost ordering's bins,
+// keys and slot map (up to kNoAllocMaxOrdered orbits), and the Milky-Way force
+// tables of the last MW potential used on that GPU.  Host code never passes
+// the array's name to a kernel (on the host it names a stand-in, not device
+// memory); it takes each device's address once from hipGetSymbolAddress.
+// ---------------------------------------------------------------------------
+
+constexpr size_t kNoAllocMaxOrdered = size_t(1) << 20;
+
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t kNaQueue = 0;
+constexpr size_t kNaBins = 256;
+constexpr size_t kNaKeys = align256(kNaBins + kOrderBins * sizeof(uint32_t));
+constexpr size_t kNaSlots = align256(kNaKeys + kNoAllocMaxOrdered * sizeof(uint16_t));
+constexpr size_t kNaTable64 = align256(kNaSlots + kNoAllocMaxOrdered * sizeof(uint32_t));
+constexpr size_t kNaTable32 = align256(kNaTable64 + TABLE_DOUBLES * sizeof(double));
+constexpr size_t kNaFar = align256(kNaTable32 + TABLE32_FLOATS * sizeof(float));
+constexpr size_t kNoAllocScratchBytes = align256(kNaFar + FAR_DOUBLES * sizeof(double));
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+struct NoAllocDevice {
+    bool ready = false;
+    unsigned char* base = nullptr;  // this device's copy of `scratch`
+    int cu_count = 0;
+    bool have_table = false;        // the MW tables of table_key are resident
+    drift_potential table_key{};
+};
+
+constexpr int kNoAllocMaxDevices = 64;
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+// guards g_noalloc and the host staging of a table being built
+std::mutex g_noalloc_mutex;
+double g_noalloc_h64[TABLE_DOUBLES];
+float g_noalloc_h32[TABLE32_FLOATS];
+double g_noalloc_far[FAR_DOUBLES];
+
+// Once per device, with that device current: the device address of its
+// scratch array and its CU count.  Caller holds g_noalloc_mutex.
+int noalloc_setup(int device, NoAllocDevice*& out) {
+    if (device < 0 || device >= kNoAllocMaxDevices)
+        return fail(DRIFT_ERR_UNSUPPORTED, "device index beyond the allocation-free entry "
+                                           "point's " + std::to_string(kNoAllocMaxDevices) +
+                                               " devices");
+    NoAllocDevice& d = g_noalloc[device];
+    if (!d.ready) {
+        void* ptr = nullptr;
+        DRIFT_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess)
+            cus = 256;
+        d.base = static_cast<unsigned char*>(ptr);
+        d.cu_count = cus;
+        d.ready = true;
+    }
+    out = &d;
+    return DRIFT_OK;
+}
+
+// Makes the MW tables of p resident in the device's scratch.  A change of
+// potential waits for the device (a call still running may read the old
+// tables) and copies synchronously; it is refused under stream capture.
+int noalloc_tables(NoAllocDevice& d, const drift_potential& p, bool capturing) {
+    drift_potential key = p;
+    key.origins = 0;
+    if (d.have_table && std::memcmp(&d.table_key, &key, sizeof key) == 0) return DRIFT_OK;
+    if (capturing)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                    "stream capture: the Milky-Way tables of this potential are not resident; "
+                    "make one call with it outside the capture first");
+    build_mw_table(p, g_noalloc_h64);
+    build_mw_table_f32(p, g_noalloc_h32);
+    build_mw_far_table(p, g_noalloc_far);
+    d.have_table = false;
+    DRIFT_HIP_TRY(hipDeviceSynchronize());
+    DRIFT_HIP_TRY(hipMemcpy(d.base + kNaTable64, g_noalloc_h64, sizeof g_noalloc_h64,
+                            hipMemcpyHostToDevice));
+    DRIFT_HIP_TRY(hipMemcpy(d.base + kNaTable32, g_noalloc_h32, sizeof g_noalloc_h32,
+                            hipMemcpyHostToDevice));
+    DRIFT_HIP_TRY(hipMemcpy(d.base + kNaFar, g_noalloc_far, sizeof g_noalloc_far,
+                            hipMemcpyHostToDevice));
+    d.table_key = key;
+    d.have_table = true;
+    return DRIFT_OK;
+}
+
+int noalloc_dop853(const double* d_state0, const double* d_times, double* d_out, size_t n,
+                   size_t nt, double rtol_ln, double atol_ln, const drift_potential* potential,
+                   drift_orbit_stats* d_stats, const drift_launch_options* options,
+                   hipStream_t st) {
+    if (n < 1) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n must be >= 1");
+    if (nt < 2) return fail(DRIFT_ERR_INVALID_ARGUMENT, "nt must be >= 2");
+    if (nt > (size_t)INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "nt too large");
+    if (!d_state0 || !d_times || !d_out)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "state0, times and state_out must be non-NULL");
+    if ((reinterpret_cast<uintptr_t>(d_state0) | reinterpret_cast<uintptr_t>(d_out)) & 15u)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "state0 and state_out must be 16-byte aligned");
+    if (!finite(rtol_ln) || !finite(atol_ln))
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "rtol and atol (natural logs) must be finite");
+    drift_launch_options opt{};
+    if (options) opt = *options;
+    if (opt.precision != DRIFT_PRECISION_FP64 && opt.precision != DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "unknown precision mode");
+    const int block = opt.block_size ? opt.block_size : kDefaultBlock;
+    if (block < 64 || block > kDefaultBlock || block % 64 != 0)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
+    if (opt.schedule != DRIFT_SCHEDULE_PLAIN && opt.schedule != DRIFT_SCHEDULE_PERSISTENT)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "the allocation-free entry point takes PLAIN or PERSISTENT (tuned choices "
+                    "live in a session)");
+    if (opt.ordering < DRIFT_ORDER_AUTO || opt.ordering > DRIFT_ORDER_COST)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT, "ordering must be AUTO, INDEX or COST");
+    if (opt.staging != DRIFT_STAGING_AUTO && opt.staging != DRIFT_STAGING_DIRECT)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "the allocation-free entry point writes its output directly (staging needs a "
+                    "buffer of the output's size)");
+    drift_potential p{};
+    p.kind = POT_GALPY_KEPLER;
+    if (potential) p = *potential;
+    int rc = validate_potential(p);
+    if (rc != DRIFT_OK) return rc;
+    if (p.origins != 0)
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "moving-origin sets belong to a session; use drift_galpy_dop853_device");
+    const bool cost_order = opt.ordering == DRIFT_ORDER_COST ||
+                            (opt.ordering == DRIFT_ORDER_AUTO && n >= kAutoOrderMinOrbits &&
+                             n <= kNoAllocMaxOrdered);
+    if (cost_order && n > kNoAllocMaxOrdered)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                    "cost ordering in the allocation-free entry point takes at most " +
+                        std::to_string(kNoAllocMaxOrdered) +
+                        " orbits (its scratch size); pass DRIFT_ORDER_INDEX for more");
+
+    PlainArgs a{};
+    a.state0 = d_state0;
+    a.times = d_times;
+    a.out = d_out;
+    a.stats = d_stats;
+    a.n = (int64_t)n;
+    a.nt = (int32_t)nt;
+    a.max_attempts = opt.max_attempts ? opt.max_attempts : kDefaultMaxAttempts;
+    a.rtol = std::exp(rtol_ln);  // glibc exp, dop853.rs:340-341
+    a.atol = std::exp(atol_ln);
+    a.dt_one_init = dp54::initial_step_sentinel;
+    dopr54_bounds(a.bound2);
+    if (opt.precision == DRIFT_PRECISION_MIXED && a.rtol < kMixedMinRtol * (1.0 - 1e-9))
+        return fail(DRIFT_ERR_UNSUPPORTED,
+                    "mixed precision (FP32 stage forces) requires rtol >= 1e-7");
+    if (p.kind == POT_NONE && opt.precision == DRIFT_PRECISION_MIXED)
+        return fail(DRIFT_ERR_UNSUPPORTED, "the NONE potential needs FP64 precision");
+
+    int device = 0;
+    DRIFT_HIP_TRY(hipGetDevice(&device));
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    DRIFT_HIP_TRY(hipStreamIsCapturing(st, &capture));
+    const bool capturing = capture != hipStreamCaptureStatusNone;
+
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    NoAllocDevice* d = nullptr;
+    if ((rc = noalloc_setup(device, d)) != DRIFT_OK) return rc;
+    unsigned char* base = d->base;
+    a.pot = PotParams{};
+    a.pot.kind = p.kind;
+    if (p.kind == POT_KEPLER) {
+        a.pot.amp = p.amp;
+    } else if (p.kind == POT_PLUMMER) {
+        a.pot.amp = p.amp;
+        a.pot.b2 = p.b * p.b;
+    } else if (p.kind == POT_NONE) {
+        // free motion: the moving-origin kernel with no origins
+        a.pot.kind = POT_MOVING | POT_NONE;
+        a.pot.n_segments = 1;
+    } else if (p.kind == POT_MW) {
+        if ((rc = noalloc_tables(*d, p, capturing)) != DRIFT_OK) return rc;
+        build_mw_params(p, a.pot);
+        a.pot.far_table = reinterpret_cast<const double*>(base + kNaFar);
+        a.table = opt.precision == DRIFT_PRECISION_MIXED
+                      ? static_cast<const void*>(base + kNaTable32)
+                      : static_cast<const void*>(base + kNaTable64);
+    }
+    PlainKernel k = kernel_for(DRIFT_METHOD_DOP853, a.pot.kind, opt.precision, opt.schedule);
+    int64_t blocks = 0;
+    if ((rc = integrator_grid(d->cu_count, k, opt.schedule, block, n, blocks)) != DRIFT_OK)
+        return rc;
+    if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");
+    if (opt.schedule == DRIFT_SCHEDULE_PERSISTENT) {
+        a.queue = reinterpret_cast<unsigned long long*>(base + kNaQueue);
+        // a fill kernel, not a memset, so the sequence may be captured
+        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, st, a.queue);
+        DRIFT_HIP_TRY(hipGetLastError());
+    }
+    if (cost_order) {
+        uint32_t* slots = reinterpret_cast<uint32_t*>(base + kNaSlots);
+        if ((rc = launch_cost_order(a, opt.precision, reinterpret_cast<uint16_t*>(base + kNaKeys),
+                                    reinterpret_cast<uint32_t*>(base + kNaBins), slots, st)) !=
+            DRIFT_OK)
+            return rc;
+        a.order = slots;
+    }
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, st, a);
+    DRIFT_HIP_TRY(hipGetLastError());
+    return DRIFT_OK;
+}
+
+}  // namespace
 }  // namespace drift
 
 using namespace drift;
@@ -1131,6 +1353,15 @@ int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
                              (hipStream_t)stream);
 }
 
+int drift_galpy_dop853_device_noalloc(const double* d_state0, const double* d_times,
+                                      double* d_state_out, size_t n, size_t nt, double rtol,
+                                      double atol, const drift_potential* potential,
+                                      drift_orbit_stats* d_stats,
+                                      const drift_launch_options* options, void* stream) {
+    return noalloc_dop853(d_state0, d_times, d_state_out, n, nt, rtol, atol, potential, d_stats,
+                          options, (hipStream_t)stream);
+}
+
 int drift_launch_shape(drift_session* s, int32_t method, const drift_potential* potential,
                        size_t n, const drift_launch_options* options, int32_t* grid,
                        int32_t* block) {
@@ -1167,8 +1398,8 @@ int drift_launch_shape(drift_session* s, int32_t method, const drift_potential* 
         return fail(DRIFT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 256]");
     DRIFT_HIP_TRY(hipSetDevice(s->device));
     int64_t blocks = 0;
-    rc = integrator_grid(s, kernel_for(method, kind, opt.precision, opt.schedule), opt.schedule, b,
-                         n, blocks);
+    rc = integrator_grid(s->cu_count, kernel_for(method, kind, opt.precision, opt.schedule),
+                         opt.schedule, b, n, blocks);
     if (rc != DRIFT_OK) return rc;
     if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");
     *grid = (int32_t)blocks;

@@ -226,6 +226,26 @@ int drift_galpy_dop853_device(drift_session* session, const double* d_state0,
                               drift_orbit_stats* d_stats, const drift_launch_options* options,
                               void* stream);
 
+/* DOP853 on device pointers without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as drift_galpy_dop853_device, on the
+ * current HIP device and `stream` (NULL = the null stream), with no session:
+ * the work-queue counter, the cost ordering's scratch and the Milky-Way force
+ * tables live in one fixed library array per GPU (about 9 MB, held until the
+ * process ends).  Limits: cost ordering takes at most 2^20 orbits (AUTO
+ * orders 16384..2^20 orbits by cost, others by index; larger COST launches
+ * are refused); schedule PLAIN or PERSISTENT; staging AUTO or DIRECT; no
+ * moving origins.  The array keeps the last MW potential's tables: a call
+ * with another MW potential waits for the device and uploads them, which is
+ * refused under stream capture.  Calls that overlap on one GPU (other
+ * streams, other threads) share the array and race: serialise them. */
+int drift_galpy_dop853_device_noalloc(const double* d_state0, const double* d_times,
+                                      double* d_state_out, size_t n, size_t nt, double rtol,
+                                      double atol, const drift_potential* potential,
+                                      drift_orbit_stats* d_stats,
+                                      const drift_launch_options* options, void* stream);
+
 /* ---- launch diagnostics (the reference's debug records of
  *      src/dispatch/gpu.rs:157-178 and gpu/cuda_oxide.rs:15-34, 77-84) -------- */
 
