@@ -354,22 +354,58 @@ __global__ void __launch_bounds__(256) unstage_kernel(const double* __restrict__
 
 // ---------------------------------------------------------------------------
 // Cost ordering: a counting sort of the orbits by a bucket of their estimated
-// orbital frequency at t0, highest first.  omega^4 = |a|^2 / r^2, bucketed by
-// half-octaves of omega^4 (1/8 octave of omega).  Orbits of similar step
-// counts then share wavefronts, and the long ones start first.
+// step rate at t0, highest first, so that orbits of similar step counts share
+// wavefronts and the long ones start first.  The rate is the orbital
+// frequency omega = sqrt(|a| / r).  For the plain schedule, whose wavefronts
+// last as long as their slowest orbit, the rate of an orbit in a Milky-Way
+// potential also counts its disk crossings: an orbit that leaves the disk
+// plane crosses the thin disk at up to v_zmax = sqrt(2 E_z) and takes short
+// steps there, so its rate adds a quarter of the crossing rate v_zmax / b (b
+// the disk's scale height).  On the 1M-orbit disk workload the mean attempts
+// of a 64-orbit wavefront over its slowest orbit's rose from 0.755 (omega
+// alone) to 0.833 at rtol 1e-12 and from 0.666 to 0.817 at 1e-7, and the
+// plain launch took 134 ms instead of 144 ms.  The persistent schedule
+// refills lanes, so only the order of claims matters to it; there omega
+// alone groups orbits whose steps emit samples equally often (fewer lanes
+// idle in the dense-output branch) and measured 1.5-2.7% faster at rtol 1e-7.
+// Buckets are 1/16 octave of the rate.
 // ---------------------------------------------------------------------------
 constexpr int kOrderBins = 1024;
+constexpr int kOrderBinsPerOctave = 16;
+constexpr double kCrossingWeight = 0.25;
 
-__device__ __forceinline__ uint32_t cost_bin(const double* y, const double* d) {
+// Vertical energy E_z = v_z^2 / 2 + Phi(R, z) - Phi(R, 0) of a Milky-Way
+// orbit: the Miyamoto-Nagai part in closed form, the spherical part as
+// S z^2 / 2 with S from the spherical force's z component -S z (the total
+// force d less the disk's).
+__device__ __forceinline__ double mw_vertical_energy(const PotParams& p, const double* y,
+                                                     const double* d) {
+    const double R2 = y[0] * y[0] + y[1] * y[1], z = y[2];
+    const double zb = std::sqrt(z * z + p.disk_b2), b = std::sqrt(p.disk_b2);
+    const double D = std::sqrt(R2 + (p.disk_a + zb) * (p.disk_a + zb));
+    const double D0 = std::sqrt(R2 + (p.disk_a + b) * (p.disk_a + b));
+    const double dphi_disk = p.disk_amp * (1.0 / D0 - 1.0 / D);
+    const double az_disk = zb > 0.0 ? -p.disk_amp * z * (p.disk_a + zb) / (zb * D * D * D) : 0.0;
+    const double dphi_sph = -0.5 * (d[5] - az_disk) * z;
+    return 0.5 * y[5] * y[5] + dphi_disk + dphi_sph;
+}
+
+template <int KIND>
+__device__ __forceinline__ uint32_t cost_bin(const PotParams& p, const double* y, const double* d,
+                                             bool crossings) {
     const double r2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
     const double a2 = d[3] * d[3] + d[4] * d[4] + d[5] * d[5];
-    const double q = a2 / r2;
+    double rate = std::sqrt(std::sqrt(a2 / r2));
+    if constexpr (static_kind(KIND) == POT_MW) {
+        if (crossings && p.disk_b2 > 0.0) {
+            const double ez = mw_vertical_energy(p, y, d);
+            if (ez > 0.0) rate += kCrossingWeight * std::sqrt(2.0 * ez / p.disk_b2);
+        }
+    }
     // no force, a singular or non-finite start: cheapest bin (such orbits end
     // at once or are free drifts)
-    if (!(q > 0.0) || !(q < INFINITY)) return kOrderBins - 1;
-    int e;
-    const double m = std::frexp(q, &e);  // q = m 2^e, m in [0.5, 1)
-    const int level = 2 * e + (m >= 0.70710678118654752 ? 1 : 0);
+    if (!(rate > 0.0) || !(rate < INFINITY)) return kOrderBins - 1;
+    const int level = (int)std::floor(kOrderBinsPerOctave * std::log2(rate));
     const int bin = kOrderBins / 2 - 1 - level;
     return (uint32_t)std::min(std::max(bin, 0), kOrderBins - 1);
 }
@@ -380,7 +416,8 @@ __global__ void order_reset_kernel(uint32_t* bins) {
 
 template <int KIND, class FT>
 __global__ void __launch_bounds__(kDefaultBlock) order_key_kernel(PlainArgs a, uint16_t* keys,
-                                                                   uint32_t* bins) {
+                                                                   uint32_t* bins,
+                                                                   int32_t crossings) {
     __shared__ FT lds[kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     __shared__ uint32_t local[kOrderBins];
     for (int k = threadIdx.x; k < kOrderBins; k += blockDim.x) local[k] = 0u;
@@ -392,7 +429,7 @@ __global__ void __launch_bounds__(kDefaultBlock) order_key_kernel(PlainArgs a, u
         double y[DIM], d[DIM];
         load_state(a.state0, i, y);
         f(a.times[0], y, d);
-        const uint32_t b = cost_bin(y, d);
+        const uint32_t b = cost_bin<KIND>(a.pot, y, d, crossings != 0);
         keys[i] = (uint16_t)b;
         atomicAdd(&local[b], 1u);
     }
@@ -685,7 +722,7 @@ PlainKernel kernel_for(int method, int kind, int precision, int schedule) {
     }
 }
 
-using KeyKernel = void (*)(PlainArgs, uint16_t*, uint32_t*);
+using KeyKernel = void (*)(PlainArgs, uint16_t*, uint32_t*, int32_t);
 
 template <int KIND>
 KeyKernel pick_key(int precision) {
@@ -766,14 +803,15 @@ int integrator_grid(int cu_count, PlainKernel k, int schedule, int block, size_t
 // itself.
 constexpr size_t kAutoOrderMinOrbits = 16384;
 
-int launch_cost_order(const PlainArgs& a, int precision, uint16_t* keys, uint32_t* bins,
-                      uint32_t* slots, hipStream_t st);
+int launch_cost_order(const PlainArgs& a, int precision, int schedule, uint16_t* keys,
+                      uint32_t* bins, uint32_t* slots, hipStream_t st);
 
 // Builds s->order_slots (slot -> orbit, most expensive bin first) on stream
 // st.  The launch that reads it records s->order_done after its integration
 // kernel; the next build waits for that event, so launches on different
 // streams never overwrite a map still in use.
-int build_cost_order(drift_session* s, const PlainArgs& a, int precision, hipStream_t st) {
+int build_cost_order(drift_session* s, const PlainArgs& a, int precision, int schedule,
+                     hipStream_t st) {
     const size_t n = (size_t)a.n;
     int rc;
     if ((rc = ensure_buffer(s->order_keys, n * sizeof(uint16_t))) != DRIFT_OK) return rc;
@@ -783,20 +821,22 @@ int build_cost_order(drift_session* s, const PlainArgs& a, int precision, hipStr
         DRIFT_HIP_TRY(hipEventCreateWithFlags(&s->order_done, hipEventDisableTiming));
     else
         DRIFT_HIP_TRY(hipStreamWaitEvent(st, s->order_done, 0));
-    return launch_cost_order(a, precision, static_cast<uint16_t*>(s->order_keys.ptr),
+    return launch_cost_order(a, precision, schedule, static_cast<uint16_t*>(s->order_keys.ptr),
                              static_cast<uint32_t*>(s->order_bins.ptr),
                              static_cast<uint32_t*>(s->order_slots.ptr), st);
 }
 
 // The four kernels of the cost ordering on caller-given scratch: keys (n
-// u16), bins (kOrderBins u32) and the slot -> orbit map (n u32).
-int launch_cost_order(const PlainArgs& a, int precision, uint16_t* keys, uint32_t* bins,
-                      uint32_t* slots, hipStream_t st) {
+// u16), bins (kOrderBins u32) and the slot -> orbit map (n u32).  The key
+// counts disk crossings for the plain schedule only.
+int launch_cost_order(const PlainArgs& a, int precision, int schedule, uint16_t* keys,
+                      uint32_t* bins, uint32_t* slots, hipStream_t st) {
+    const int32_t crossings = schedule == DRIFT_SCHEDULE_PLAIN ? 1 : 0;
     const size_t n = (size_t)a.n;
     const unsigned blocks = (unsigned)((n + kDefaultBlock - 1) / kDefaultBlock);
     hipLaunchKernelGGL(order_reset_kernel, dim3(1), dim3(256), 0, st, bins);
     hipLaunchKernelGGL(key_kernel_for(a.pot.kind, precision), dim3(blocks), dim3(kDefaultBlock), 0,
-                       st, a, keys, bins);
+                       st, a, keys, bins, crossings);
     hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(kOrderBins), 0, st, bins);
     const unsigned scatter_blocks = (unsigned)((n + kScatterItems - 1) / kScatterItems);
     hipLaunchKernelGGL(order_scatter_kernel, dim3(scatter_blocks), dim3(kScatterThreads), 0, st,
@@ -927,7 +967,7 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     if (cost_order) {
         if (n > (size_t)UINT32_MAX)
             return fail(DRIFT_ERR_INVALID_ARGUMENT, "cost ordering supports n < 2^32");
-        if ((rc = build_cost_order(s, a, opt.precision, st)) != DRIFT_OK) return rc;
+        if ((rc = build_cost_order(s, a, opt.precision, opt.schedule, st)) != DRIFT_OK) return rc;
         a.order = static_cast<const uint32_t*>(s->order_slots.ptr);
     }
     const bool staged = opt.staging == DRIFT_STAGING_STAGED;
@@ -1243,7 +1283,8 @@ int noalloc_dop853(const double* d_state0, const double* d_times, double* d_out,
     }
     if (cost_order) {
         uint32_t* slots = reinterpret_cast<uint32_t*>(base + kNaSlots);
-        if ((rc = launch_cost_order(a, opt.precision, reinterpret_cast<uint16_t*>(base + kNaKeys),
+        if ((rc = launch_cost_order(a, opt.precision, opt.schedule,
+                                    reinterpret_cast<uint16_t*>(base + kNaKeys),
                                     reinterpret_cast<uint32_t*>(base + kNaBins), slots, st)) !=
             DRIFT_OK)
             return rc;
