@@ -35,9 +35,19 @@ namespace drift {
 // table of the launch's most used MW potential is staged in LDS (`lds`,
 // in_lds set for its jobs); other MW potentials read theirs from global
 // memory.
+//
+// Jobs are grouped by potential kind, so nearly every wavefront carries one
+// kind.  set_wave_force records, after every claim, that kind when all the
+// wavefront's active lanes share it (wave_kind, a wavefront-uniform value, -1
+// otherwise) and whether all their MW tables are the LDS one (wave_lds); the
+// kernels then run the whole attempt with that kind's force type
+// (attempt_by_kind), the code of the single-job kernels, and only a wavefront
+// that straddles two groups runs the per-lane dispatch of operator().
 template <class FT>
 struct LaneForce {
     int32_t kind;
+    int32_t wave_kind;  // the wavefront's common kind, or -1
+    int32_t wave_lds;   // 1 when every active lane's MW table is the LDS one
     bool in_lds;
     FT c0, c1, c2;
     const FT* table;
@@ -58,6 +68,26 @@ struct LaneForce {
         }
     }
 };
+
+// Records the wavefront's common kind and table (see LaneForce).
+template <class FT>
+__device__ __forceinline__ void set_wave_force(LaneForce<FT>& f) {
+    const int32_t first = __builtin_amdgcn_readfirstlane(f.kind);
+    f.wave_kind = __ballot(f.kind != first) == 0 ? first : -1;
+    f.wave_lds = __ballot(!f.in_lds) == 0 ? 1 : 0;
+}
+
+// One attempt with the force type of the wavefront's common kind (scalar
+// branches: wave_kind and wave_lds are wavefront-uniform), or with the
+// per-lane dispatch.  step(force) runs the attempt.
+template <class FT, class Step>
+__device__ __forceinline__ bool attempt_by_kind(const LaneForce<FT>& f, const Step& step) {
+    if (f.wave_kind == POT_MW && f.wave_lds) return step(MwForce<FT>{f.p, f.lds, f.c0, f.c1, f.c2});
+    if (f.wave_kind == POT_KEPLER) return step(KeplerForce<FT>{f.c0});
+    if (f.wave_kind == POT_PLUMMER) return step(PlummerForce<FT>{f.c0, f.c1});
+    if (f.wave_kind == POT_GALPY_KEPLER) return step(GalpyKeplerForce<FT>{});
+    return step(f);
+}
 
 // Sets f to job potential `pot` (f.lds stays the block's LDS table).
 template <class FT>
@@ -250,6 +280,7 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dop853_kernel(R
             J = make_dop853_job(d.times, d.nt, d.rtol, d.atol, a.max_attempts, d.times[0],
                                 d.times[d.nt - 1]);
             set_lane_force<FT>(f, a, d.pot);
+            f.wave_kind = -1;  // the init below dispatches per lane
             sink = OutSinkR{d.out, d.n, orbit};
             stats = d.stats;
             done_count = d.done_count;
@@ -266,7 +297,9 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dop853_kernel(R
             }
             need = false;
         }
-        const bool done = dop853_attempt(L, J, f, sink);
+        set_wave_force(f);
+        const bool done = attempt_by_kind(
+            f, [&](const auto& force) { return dop853_attempt(L, J, force, sink); });
         if (done || L.ii >= stop) {
             if (res.base) save(res, L, done ? kOrbitDone : kOrbitRunning);
             if (done) {
@@ -318,6 +351,7 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dopr54_kernel(R
             J.atol = d.atol;
             J.dt_one_init = d.dt_one_init;
             set_lane_force<FT>(f, a, d.pot);
+            f.wave_kind = -1;  // the init below dispatches per lane
             sink = OutSinkR{d.out, d.n, orbit};
             stats = d.stats;
             done_count = d.done_count;
@@ -334,7 +368,10 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dopr54_kernel(R
             }
             need = false;
         }
-        if (!done) done = dopr54_attempt(L, J, f, sink);
+        set_wave_force(f);
+        if (!done)
+            done = attempt_by_kind(
+                f, [&](const auto& force) { return dopr54_attempt(L, J, force, sink); });
         if (done || L.ii >= stop) {
             if (res.base) save(res, L, done ? kOrbitDone : kOrbitRunning);
             if (done) {

@@ -59,6 +59,8 @@ def run(args) -> int:
         del a, b
         torch.cuda.empty_cache()
 
+    if args.ragged_jobs > 0:
+        return run_ragged(args.ragged_jobs)
     w = workloads.mw_disk(n=args.n)
     s0 = torch.from_numpy(w.state0).to(dev)
     tt = torch.from_numpy(w.times).to(dev)
@@ -72,6 +74,66 @@ def run(args) -> int:
     print(f"[profiling] integrator: n={w.n} nt={w.nt} accepted={acc} "
           f"algorithmic bytes={algorithmic_bytes(w.n, w.nt)}", flush=True)
     return 0
+
+
+def run_ragged(njobs: int) -> int:
+    """One drift_ragged_device call on the first njobs jobs of the ragged mix
+    (256-thread blocks, no tuner; one launch per method), inputs and outputs
+    resident in HBM.  Prints each method's algorithmic FLOPs."""
+    import numpy as np
+    import torch
+
+    from . import _native, flops, workloads
+
+    dev = torch.device("cuda:0")
+    jobs = workloads.ragged_mix(njobs)
+    pots = list(workloads.RAGGED_POTENTIALS)
+    keep, descs = [], []
+    for j in jobs:
+        s0 = torch.from_numpy(j.state0).to(dev)
+        tt = torch.from_numpy(j.times).to(dev)
+        out = torch.empty((j.nt, j.n, 6), dtype=torch.float64, device=dev)
+        st = torch.zeros((j.n, 4), dtype=torch.int32, device=dev)
+        keep.append((s0, tt, out, st))
+        lt = math.log(j.rtol)
+        method = _native.METHOD_DOP853 if j.method == "DOP853" else _native.METHOD_DOPR54
+        descs.append(_native.DeviceJob(method, j.potential, j.n, j.nt, lt, lt, s0.data_ptr(),
+                                       tt.data_ptr(), out.data_ptr(), st.data_ptr(),
+                                       span_hint=float(j.times[-1] - j.times[0])))
+    torch.cuda.synchronize()
+    _native.ragged_device(descs, pots, block_size=256)
+    torch.cuda.synchronize()
+    flop = {"DOP853": 0.0, "DOPR54": 0.0}
+    for j, (_, _, _, st) in zip(jobs, keep):
+        s = st.cpu().numpy()
+        spec = pots[j.potential]
+        flop[j.method] += (flops.dop853_flops(s, j.nt, spec) if j.method == "DOP853"
+                           else flops.dopr54_flops(s, spec))
+    print(f"[profiling] ragged mix: {njobs} jobs, {sum(j.n for j in jobs)} orbits, algorithmic "
+          f"FLOPs DOP853 {flop['DOP853']:.4e} DOPR54 {flop['DOPR54']:.4e}", flush=True)
+    return 0
+
+
+def derived(c: dict[str, float]) -> list[tuple[str, str, str]]:
+    """Ratios of an SQ counter set (whichever inputs are present)."""
+    rows = []
+    if "SQ_INSTS_SALU" in c and "SQ_INSTS_VALU" in c:
+        rows.append(("SALU per VALU instruction", f"{c['SQ_INSTS_SALU'] / c['SQ_INSTS_VALU']:.3f}",
+                     "INSTS_SALU / INSTS_VALU"))
+    if "SQ_ACTIVE_INST_VALU" in c and "SQ_WAVE_CYCLES" in c:
+        rows.append(("VALU busy", f"{100 * c['SQ_ACTIVE_INST_VALU'] / (c['SQ_WAVE_CYCLES'] / 2):.1f}%",
+                     "ACTIVE_INST_VALU / (WAVE_CYCLES / 2 waves per SIMD)"))
+    if "SQ_THREAD_CYCLES_VALU" in c and "SQ_ACTIVE_INST_VALU" in c:
+        rows.append(("active lanes per VALU instruction",
+                     f"{100 * c['SQ_THREAD_CYCLES_VALU'] / (64 * c['SQ_ACTIVE_INST_VALU']):.1f}%",
+                     "THREAD_CYCLES_VALU / (64 ACTIVE_INST_VALU)"))
+    f64 = [k for k in ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64",
+                       "SQ_INSTS_VALU_TRANS_F64") if k in c]
+    if len(f64) == 4 and "SQ_INSTS_VALU" in c:
+        tot = sum(c[k] for k in f64)
+        rows.append(("FP64 share of VALU instructions", f"{100 * tot / c['SQ_INSTS_VALU']:.1f}%",
+                     "(FMA + MUL + ADD + TRANS F64) / INSTS_VALU"))
+    return rows
 
 
 def _csv_rows(d: Path, suffix: str) -> list[dict]:
@@ -136,6 +198,32 @@ def long_dispatches(trace: Path, min_ms: float) -> None:
         print(f"| {k} | `{short}` | {ms:.3f} |")
 
 
+def kernel_counters(dirs: list[str], needle: str) -> int:
+    """SQ counters of every dispatch whose name contains `needle`, per counter
+    folder, merged per dispatch order, with derived ratios."""
+    merged: dict[str, dict[str, float]] = {}
+    for d in dirs:
+        for k, c in _counters(Path(d)).items():
+            if needle in k:
+                merged.setdefault(k.split(" [dispatch")[0], {}).update(c)
+    if not merged:
+        raise KeyError(f"no dispatch matching {needle}")
+    for name, c in merged.items():
+        print(f"Counters of `{_short(name)}`:")
+        print()
+        print("| counter | value |")
+        print("|---|---|")
+        for k, v in sorted(c.items()):
+            print(f"| {k} | {v:.6e} |")
+        print()
+        print("| quantity | value | how |")
+        print("|---|---|---|")
+        for q, v, how in derived(c):
+            print(f"| {q} | {v} | {how} |")
+        print()
+    return 0
+
+
 def summarize(dirs: list[str], min_ms: float = 0.0) -> int:
     trace = Path(dirs[0])
     if min_ms > 0.0:
@@ -196,10 +284,18 @@ def main() -> int:
     p.add_argument("--summarize", nargs="+", metavar="DIR",
                    help="trace folder, then optionally the fetch and write folders and more "
                         "counter folders")
+    p.add_argument("--ragged-jobs", type=int, default=0,
+                   help="run the first N jobs of the ragged mix (one drift_ragged_device call) "
+                        "instead of the 1M-orbit launch")
+    p.add_argument("--kernel", metavar="NAME",
+                   help="with --summarize: the SQ counters of the dispatches whose name contains "
+                        "NAME, from every folder given, with derived ratios")
     p.add_argument("--min-ms", type=float, default=0.0,
                    help="with --summarize and a trace folder alone: only dispatches of at "
                         "least this many ms, grouped and in launch order")
     args = p.parse_args()
+    if args.summarize and args.kernel:
+        return kernel_counters(args.summarize, args.kernel)
     if args.summarize:
         if len(args.summarize) == 2:
             p.error("--summarize takes the trace folder, then the fetch and write folders")

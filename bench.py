@@ -80,17 +80,20 @@ def profiled_traffic(n: int, schedule: str, tuned: dict | None):
 
 
 def ragged_leg(njobs: int):
-    """The first `njobs` jobs of the ragged mix through drift_ragged_device
-    (one persistent launch per integration method), inputs and outputs
-    resident in HBM.  Jobs whose outputs together exceed 70% of the free HBM
-    run as consecutive batches (the full 4,096-job mix writes 371 GB); the
-    rate is accepted particle-steps over the summed batch times."""
+    """The first `njobs` jobs of the ragged mix through drift_ragged_device,
+    inputs and outputs resident in HBM.  Jobs whose outputs together exceed
+    70% of the free HBM run as consecutive batches (the full 4,096-job mix
+    writes 371 GB).  Each batch runs its DOP853 jobs and then its DOPR54 jobs
+    as two calls (the library makes one persistent launch per method either
+    way), so each method's time and its algorithmic FP64 FLOP/s (from the
+    per-orbit statistics) are reported beside the whole mix's rate."""
     import numpy as np
     import torch
 
-    from astrodrift_amd import _native, workloads
+    from astrodrift_amd import _native, flops, workloads
 
     jobs = workloads.ragged_mix(njobs)
+    pots = list(workloads.RAGGED_POTENTIALS)
     dev = torch.device("cuda:0")
     torch.cuda.empty_cache()
     budget = 0.7 * torch.cuda.mem_get_info(dev)[0]
@@ -105,6 +108,8 @@ def ragged_leg(njobs: int):
     batches.append(cur)
     acc = rej = bad = 0
     seconds = []
+    methods = {m: {"seconds": 0.0, "accepted": 0, "flop": 0.0, "jobs": 0}
+               for m in ("DOP853", "DOPR54")}
     tuned = None
     for batch in batches:
         keep, descs = [], []
@@ -123,17 +128,32 @@ def ragged_leg(njobs: int):
         if tuned is None:
             # block size per method from a 5% prefix of the first batch's grids,
             # outside the timed region; later launches use it
-            tuned = _native.tune_ragged(descs, list(workloads.RAGGED_POTENTIALS))
+            tuned = _native.tune_ragged(descs, pots)
             print(f"[bench] ragged tuned blocks: {tuned}", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        _native.ragged_device(descs, list(workloads.RAGGED_POTENTIALS))
-        torch.cuda.synchronize()
-        seconds.append(time.perf_counter() - t0)
-        for (_, _, _, st) in keep:
+        batch_s = 0.0
+        for name in ("DOP853", "DOPR54"):
+            sub = [d for d, j in zip(descs, batch) if j.method == name]
+            if not sub:
+                continue
+            t0 = time.perf_counter()
+            _native.ragged_device(sub, pots)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            methods[name]["seconds"] += dt
+            batch_s += dt
+        seconds.append(batch_s)
+        for j, (_, _, _, st) in zip(batch, keep):
             s = st.cpu().numpy()
-            acc += int(s[:, 0].astype(np.int64).sum())
+            a = int(s[:, 0].astype(np.int64).sum())
+            acc += a
             rej += int(s[:, 1].astype(np.int64).sum())
             bad += int(np.count_nonzero(s[:, 3] & ~0xFF))
+            m = methods[j.method]
+            m["accepted"] += a
+            m["jobs"] += 1
+            spec = pots[j.potential]
+            m["flop"] += (flops.dop853_flops(s, j.nt, spec) if j.method == "DOP853"
+                          else flops.dopr54_flops(s, spec))
         print(f"[bench] ragged batch of {len(batch)} jobs: {seconds[-1]:.3f} s", file=sys.stderr,
               flush=True)
         del keep, descs
@@ -141,12 +161,27 @@ def ragged_leg(njobs: int):
     n = sum(j.n for j in jobs)
     out_bytes = sum(48 * j.n * j.nt for j in jobs)
     wall = sum(seconds)
+    total_flop = sum(m["flop"] for m in methods.values())
+    per_method = {}
+    for name, m in methods.items():
+        if m["seconds"] > 0:
+            tf = m["flop"] / m["seconds"] / 1e12
+            per_method[name] = {"jobs": m["jobs"], "seconds": round(m["seconds"], 3),
+                                "accepted": m["accepted"], "value": m["accepted"] / m["seconds"],
+                                "tflops": tf, "frac": tf / FP64_PEAK_TFLOPS}
+    achieved = total_flop / wall / 1e12
     return {
         "jobs": njobs, "orbits": n, "output_gb": round(out_bytes / 1e9, 2),
         "dop853_jobs": sum(j.method == "DOP853" for j in jobs),
         "batches": [len(b) for b in batches], "batch_seconds": [round(x, 3) for x in seconds],
         "accepted": acc, "rejected": rej, "flagged_orbits": bad,
         "seconds": round(wall, 3), "value": acc / wall, "unit": UNIT, "tuned_blocks": tuned,
+        "per_method": per_method,
+        # algorithmic FP64 work of the whole mix (flops.py, from the per-orbit
+        # statistics) over the summed call times, against the FP64 vector peak
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "flop": total_flop},
     }
 
 
