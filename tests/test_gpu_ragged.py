@@ -241,3 +241,78 @@ def test_ragged_autotuner_choice_keeps_every_bit(gpu):
         assert np.array_equal(s, rs)
     with pytest.raises(_native.DriftNativeError, match="no jobs"):
         _native.tune_ragged([], POTS)
+
+
+def test_ragged_mix_draw_matches_oracle(gpu):
+    """The first 64 jobs of the seeded ragged mix (workloads.ragged_mix, its
+    real distribution): 16 to 61,185 orbits per job, spans up to 198 time
+    units, up to 496 outputs, rtol 1e-12 to 1e-8, all three potentials and
+    both methods, 468,039 orbits in one call.  Five orbits of every job,
+    strided over the job, against the oracle on the same inputs, the
+    reference's error-summary style of check (tests/galpy_integrator_tests.rs:
+    174-236): the parity bar of 1e-8 relative per sample, except for an orbit
+    whose trajectory the problem itself does not fix to that accuracy.  The
+    oracle run from the same start with x0 moved by one ulp measures that:
+    over 143 time units a chaotic Milky-Way orbit moves 2e-7 that way, and
+    any change of rounding (the kernels contract multiply-adds and use the
+    second-order form) moves it as much.  Such an orbit must agree within 10x
+    its one-ulp sensitivity, and at most 3% of the checked orbits may need
+    that allowance.  Step counts are equal on all but 3% of the checked
+    orbits (a step decision within a few ulps of its threshold, or on a
+    chaotic orbit, may fall the other way)."""
+    torch = pytest.importorskip("torch")
+    jobs = workloads.ragged_mix(64)
+    assert max(j.n for j in jobs) >= 32768
+    assert max(float(j.times[-1]) for j in jobs) >= 150.0
+    assert max(j.nt for j in jobs) >= 400
+    dev = torch.device("cuda:0")
+    keep, descs = [], []
+    for j in jobs:
+        s0 = torch.from_numpy(j.state0).to(dev)
+        tt = torch.from_numpy(j.times).to(dev)
+        out = torch.full((j.nt, j.n, 6), float("nan"), dtype=torch.float64, device=dev)
+        st = torch.zeros((j.n, 4), dtype=torch.int32, device=dev)
+        keep.append((s0, tt, out, st))  # the inputs must outlive the launch
+        lt = math.log(j.rtol)
+        method = _native.METHOD_DOP853 if j.method == "DOP853" else _native.METHOD_DOPR54
+        descs.append(_native.DeviceJob(method, j.potential, j.n, j.nt, lt, lt, s0.data_ptr(),
+                                       tt.data_ptr(), out.data_ptr(), st.data_ptr(),
+                                       span_hint=float(j.times[-1])))
+    print(f"ragged mix draw: {len(jobs)} jobs, {sum(j.n for j in jobs)} orbits", flush=True)
+    _native.ragged_device(descs, POTS)
+    torch.cuda.synchronize()
+    worst, checked, flips, chaotic = 0.0, 0, 0, 0
+    for k, (j, (_, _, out, st)) in enumerate(zip(jobs, keep)):
+        idx = np.unique(np.linspace(0, j.n - 1, 5).astype(np.int64))
+        got = out[:, torch.from_numpy(idx).to(dev)].cpu().numpy()
+        gst = st[torch.from_numpy(idx).to(dev)].cpu().numpy()
+        lt = math.log(j.rtol)
+        method = _native.METHOD_DOP853 if j.method == "DOP853" else _native.METHOD_DOPR54
+        ref, rst = oracle_lib.batch(method, np.ascontiguousarray(j.state0[idx]), j.times, lt, lt,
+                                    POTS[j.potential], nthreads=8)
+        assert not np.isnan(got).any(), k
+        np.testing.assert_array_equal(got[0], j.state0[idx])
+        checked += idx.size
+        flips += int(np.count_nonzero((gst[:, 0] != rst["accepted"].astype(np.int32)) |
+                                      (gst[:, 1] != rst["rejected"].astype(np.int32))))
+        for q in range(idx.size):
+            err = rel(got[:, q], ref[:, q])
+            if err < 1e-8:
+                worst = max(worst, err)
+                continue
+            # the problem's own sensitivity: the oracle from x0 moved one ulp
+            y1 = np.ascontiguousarray(j.state0[idx[q]:idx[q] + 1]).copy()
+            y1[0, 0] = np.nextafter(y1[0, 0], np.inf)
+            per, _ = oracle_lib.batch(method, y1, j.times, lt, lt, POTS[j.potential])
+            ulp = rel(per[:, 0], ref[:, q])
+            print(f"  job {k} orbit {int(idx[q])}: {j.method} pot {j.potential} "
+                  f"span={j.times[-1]:.0f} rtol={j.rtol:.1e}: rel {err:.3e}, the oracle's "
+                  f"one-ulp sensitivity {ulp:.3e}", flush=True)
+            assert err < 10.0 * ulp, (k, int(idx[q]), err, ulp)
+            chaotic += 1
+    print(f"ragged mix draw: worst rel vs oracle {worst:.3e} over {checked - chaotic} of "
+          f"{checked} orbits of {len(jobs)} jobs; {chaotic} chaotic orbits within 10x their "
+          f"one-ulp sensitivity; {flips} with different step counts", flush=True)
+    assert worst < 1e-8
+    assert chaotic <= 0.03 * checked
+    assert flips <= 0.03 * checked
