@@ -104,6 +104,9 @@ struct drift_server {
     unsigned int* done_pinned = nullptr;  // finished-orbit count of each active request
     size_t done_capacity = 0;
     std::atomic<uint64_t> graph_launches{0}, graph_captures{0}, eager_chunks{0};
+    // drift_server_busy: summed chunk times (launch to stream synchronisation)
+    // and result copies to host memory
+    std::atomic<uint64_t> busy_ns{0}, copy_ns{0}, copy_bytes{0};
 };
 
 namespace drift {
@@ -353,11 +356,12 @@ void worker_main(drift_server* S) {
         if (active.empty()) continue;
         const Clock::time_point c0 = Clock::now();
         int rc = run_chunk(S, active, st);
+        const double ms = std::chrono::duration<double, std::milli>(Clock::now() - c0).count();
+        S->busy_ns += (uint64_t)(ms * 1e6);
         if (S->opt.chunk_samples == 0 && rc == DRIFT_OK) {
             // adaptive chunk length: keep a chunk near the target duration so
             // queued requests join within about one chunk, and chunks stay long
             // enough that the pause/resume at their boundaries is cheap
-            const double ms = std::chrono::duration<double, std::milli>(Clock::now() - c0).count();
             if (ms < 0.5 * S->opt.chunk_target_ms) S->chunk_now = std::min(S->chunk_now * 2, 512);
             else if (ms > 2.0 * S->opt.chunk_target_ms) S->chunk_now = std::max(S->chunk_now / 2, 1);
         }
@@ -391,7 +395,11 @@ void finisher_main(drift_server* S) {
         }
         int status = DRIFT_OK;
         uint64_t accepted = 0;
+        const Clock::time_point f0 = Clock::now();
         const int frc = finish(*r, S->copy_stream, status, accepted);
+        S->copy_ns += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                          Clock::now() - f0).count();
+        S->copy_bytes += (uint64_t)r->req.nt * r->req.n * DIM * sizeof(double);
         complete(S, *r, frc == DRIFT_OK ? DRIFT_REQUEST_DONE : DRIFT_REQUEST_FAILED,
                  frc == DRIFT_OK ? status : frc, accepted);
     }
@@ -496,6 +504,14 @@ int drift_server_stats(drift_server* S, uint64_t* graph_launches, uint64_t* grap
     if (graph_launches) *graph_launches = S->graph_launches.load();
     if (graph_captures) *graph_captures = S->graph_captures.load();
     if (eager_chunks) *eager_chunks = S->eager_chunks.load();
+    return DRIFT_OK;
+}
+
+int drift_server_busy(drift_server* S, double* busy_s, double* copy_s, uint64_t* copy_bytes) {
+    if (!S) return fail(DRIFT_ERR_INVALID_ARGUMENT, "NULL server");
+    if (busy_s) *busy_s = S->busy_ns.load() * 1e-9;
+    if (copy_s) *copy_s = S->copy_ns.load() * 1e-9;
+    if (copy_bytes) *copy_bytes = S->copy_bytes.load();
     return DRIFT_OK;
 }
 

@@ -76,6 +76,9 @@ def _declare(lib) -> None:
     lib.drift_server_stats.restype = c.c_int
     lib.drift_server_stats.argtypes = [c.c_void_p, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64),
                                        c.POINTER(c.c_uint64)]
+    lib.drift_server_busy.restype = c.c_int
+    lib.drift_server_busy.argtypes = [c.c_void_p, c.POINTER(c.c_double), c.POINTER(c.c_double),
+                                      c.POINTER(c.c_uint64)]
     lib.drift_server_destroy.restype = c.c_int
     lib.drift_server_destroy.argtypes = [c.c_void_p]
     lib._serving_declared = True
@@ -147,6 +150,15 @@ class Server:
         _native.check(self.lib.drift_server_stats(self.handle, *[ctypes.byref(x) for x in v]))
         return {"graph_launches": v[0].value, "graph_captures": v[1].value,
                 "eager_chunks": v[2].value}
+
+    def busy(self) -> dict:
+        """drift_server_busy: seconds in chunk launches (the GPU's share of the
+        wall time), seconds and bytes of result copies to host memory."""
+        busy, copy = ctypes.c_double(), ctypes.c_double()
+        nbytes = ctypes.c_uint64()
+        _native.check(self.lib.drift_server_busy(self.handle, ctypes.byref(busy),
+                                                 ctypes.byref(copy), ctypes.byref(nbytes)))
+        return {"busy_s": busy.value, "copy_s": copy.value, "copy_bytes": nbytes.value}
 
     def close(self) -> None:
         if self.handle:

@@ -50,7 +50,8 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--serving-seconds", type=float, default=8.0,
                    help="length of the Poisson serving run (0 = skip)")
-    p.add_argument("--serving-rate", type=float, default=8.0, help="requests per second")
+    p.add_argument("--serving-rate", type=float, default=8.0,
+                   help="the light offered load, requests per second")
     p.add_argument("--ragged-jobs", type=int, default=4096,
                    help="jobs of the ragged mix to run, in HBM-sized batches (0 = skip)")
     p.add_argument("--no-precision", action="store_true",
@@ -77,6 +78,45 @@ def profiled_traffic(n: int, schedule: str, tuned: dict | None):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":", 1)[1])
     return None
+
+
+# Offered loads of the serving leg: a light one, and the heaviest the host
+# link sustains with every trajectory copied back to host memory (results of
+# 1e3-1e6 stars x 101 outputs average 0.7 GB a request; the link carries
+# about 55 GB/s into pinned memory, so FP64 at rtol 1e-12 saturates it near
+# 64 requests/s and the mixed mode at rtol 1e-7, with fewer steps per output,
+# sooner in GPU terms).
+SERVING_LOADS = (
+    ("fp64", 1e-12, 48.0),
+    ("mixed", 1e-7, 64.0),
+)
+SERVING_KEEP = ("rate_hz", "requests", "stars_mean", "p50_s", "p99_s", "max_s",
+                "queue_wait_p50_s", "queue_wait_p99_s", "throughput", "wall_s",
+                "chunk_target_ms", "gpu_busy", "copy_out_gb", "copy_out_gbps",
+                "copy_out_busy", "client_waits", "chunk_launches")
+
+
+def serving_leg(seconds: float, light_hz: float):
+    """Poisson tidal-stream requests through drift_server at two offered loads
+    per precision (FP64 at rtol 1e-12, the mixed mode at rtol 1e-7).  Results
+    land in one pinned host arena.  The chunk-target autotuner runs first on a
+    2 s prefix of the heavy load's stream and its choice serves both loads."""
+    from astrodrift_amd import _native, loadgen
+
+    arena = loadgen.PinnedArena(64 << 30)
+    out = {"pinned_arena_gb": 64 * 2**30 / 1e9}
+    for name, rtol, heavy_hz in SERVING_LOADS:
+        prec = _native.PRECISION_MIXED if name == "mixed" else _native.PRECISION_FP64
+        tuned = loadgen.tune_chunk_target(heavy_hz, rtol=rtol, precision=prec, arena=arena)
+        leg = {"rtol": rtol, "tuner": tuned}
+        for load, hz in (("light", light_hz), ("heavy", heavy_hz)):
+            r = loadgen.run_poisson(rate_hz=hz, duration_s=seconds, rtol=rtol, precision=prec,
+                                    chunk_target_ms=tuned["chunk_target_ms"], arena=arena)
+            leg[load] = {k: r[k] for k in SERVING_KEEP}
+            print(f"[bench] serving {name} {load}: {leg[load]}", file=sys.stderr, flush=True)
+        out[name] = leg
+    del arena
+    return out
 
 
 def ragged_leg(njobs: int):
@@ -563,11 +603,7 @@ def main() -> int:
                                                   r["ms_per_step"])
         print(f"[bench] moving origin: {line['moving_origin']}", file=sys.stderr, flush=True)
     if args.serving_seconds > 0:
-        from astrodrift_amd import loadgen
-
-        srv = loadgen.run_poisson(rate_hz=args.serving_rate, duration_s=args.serving_seconds)
-        line["serving"] = srv
-        print(f"[bench] serving: {srv}", file=sys.stderr, flush=True)
+        line["serving"] = serving_leg(args.serving_seconds, args.serving_rate)
     if args.ragged_jobs > 0:
         line["ragged"] = ragged_leg(args.ragged_jobs)
         print(f"[bench] ragged: {line['ragged']}", file=sys.stderr, flush=True)

@@ -40,7 +40,8 @@
 extern "C" {
 #endif
 
-/* 6: drift_launch_shape, drift_session_last_launch.
+/* 6: drift_launch_shape, drift_session_last_launch, drift_server_busy,
+ * drift_galpy_dop853_device_noalloc.
  * 5: drift_tune_ragged, drift_server_stats.  4: moving origins, drift_force,
  * the NONE potential.  3: drift_launch_options.staging.  2: .ordering.  A
  * binding checks drift_get_abi_version() against the header it was built with. */
@@ -401,6 +402,12 @@ int drift_server_wait(drift_server* server, uint64_t ticket, double timeout_s,
  * reference, which has no serving loop. */
 int drift_server_stats(drift_server* server, uint64_t* graph_launches, uint64_t* graph_captures,
                        uint64_t* eager_chunks);
+/* Where the server's time went so far: busy_s, the summed chunk launches
+ * (from the first enqueue to the chunk stream's synchronisation: the GPU's
+ * share of the wall time); copy_s and copy_bytes, the finished requests'
+ * trajectories copied to host memory (their rate is the host link's).  Any
+ * pointer may be NULL.  No counterpart in the reference. */
+int drift_server_busy(drift_server* server, double* busy_s, double* copy_s, uint64_t* copy_bytes);
 /* Finishes every submitted request, stops the loop and frees the server. */
 int drift_server_destroy(drift_server* server);
 

@@ -27,7 +27,8 @@ def test_header_declares_the_expected_entry_points():
         "drift_origin_tracks_hermite", "drift_server_create", "drift_server_submit",
         "drift_server_query", "drift_server_wait", "drift_server_stats", "drift_server_destroy",
         "drift_ragged_device", "drift_tune_ragged", "drift_tune_device",
-        "drift_launch_shape", "drift_session_last_launch",
+        "drift_launch_shape", "drift_session_last_launch", "drift_server_busy",
+        "drift_galpy_dop853_device_noalloc",
     } <= names
 
 
@@ -85,3 +86,6 @@ def test_null_handles_of_the_tuners_and_server_are_errors():
     rc = lib.drift_server_stats(None, ctypes.byref(v), None, None)
     assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT and v.value == 7
     assert "server" in _native.last_error()
+    busy = ctypes.c_double(2.5)
+    rc = lib.drift_server_busy(None, ctypes.byref(busy), None, None)
+    assert rc == _native.DRIFT_ERR_INVALID_ARGUMENT and busy.value == 2.5

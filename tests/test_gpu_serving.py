@@ -110,8 +110,13 @@ def test_requests_overlap_and_budget_admission(gpu):
         hs = [srv.submit(1, w.state0, w.times, w.rtol, w.atol, w.potential) for _ in range(6)]
         infos = [srv.wait(h, timeout_s=120) for h in hs]
         st = srv.stats()
+        busy = srv.busy()
     # the admitted set changed as requests came and went: captured anew each time
-    print(st)
+    print(st, busy)
+    # every result copied once; chunk and copy time accounted
+    assert busy["copy_bytes"] == 6 * 51 * 2000 * 6 * 8
+    assert busy["busy_s"] > 0 and busy["copy_s"] > 0
+    assert busy["busy_s"] < max(i.complete_s for i in infos) - min(i.submit_s for i in infos)
     assert st["eager_chunks"] == 0 and st["graph_captures"] >= 2
     assert st["graph_launches"] > st["graph_captures"]
     admits = sorted(i.admit_s for i in infos)
@@ -201,3 +206,21 @@ def test_full_size_tidal_stream_requests_match_oracle(gpu):
         print(f"  {w.n} stars: worst rel vs oracle {max(dp.max(), dv.max()):.3e}, identical "
               f"accepted counts on {same:.2f} of the checked stars", flush=True)
         assert dp.max() < 1e-8 and dv.max() < 1e-8
+
+
+def test_poisson_load_into_pinned_arena(gpu):
+    """The load generator's short run with results in a pinned host arena:
+    every request finishes, every arena range comes back (the free list is
+    one range again), and the copy accounting matches the results' bytes."""
+    from astrodrift_amd import loadgen
+
+    arena = loadgen.PinnedArena(8 << 30)
+    total = arena._words.size
+    print("poisson load: 16 requests/s for 1 s, 1e3-1e5 stars, pinned arena", flush=True)
+    r = loadgen.run_poisson(rate_hz=16.0, duration_s=1.0, n_max=100_000, rtol=1e-10,
+                            arena=arena)
+    print({k: r[k] for k in ("requests", "p50_s", "p99_s", "gpu_busy", "copy_out_gbps")})
+    assert r["requests"] >= 1 and r["client_waits"] == 0
+    assert arena._free == [(0, total)]
+    assert 0.0 < r["gpu_busy"] < 1.0 and r["copy_out_gb"] > 0
+    assert r["p50_s"] <= r["p99_s"] <= r["max_s"]
