@@ -310,8 +310,103 @@ DRIFT_HD DRIFT_FI void rkn853_stage(const Dop853Job& J, double t_step, const dou
     for (int c = 0; c < 3; c++) Fs[I][c] = a[3 + c];
 }
 
-// One attempted DOP853 step, dop853.rs:425-782.  Returns true when the
-// orbit has emitted its last sample.
+// Dense output of one accepted step (dop853.rs:636-763): the three extra
+// stages, rcont1..8 and every sample in (t_step, t_step + h].  Fs holds the
+// step's stage forces F1 and F6..F13 (the ones the dense stages and rcont1..8
+// read; F14..F16 are computed here), y0 the step's start and k5 its end.
+// Lc0 holds stage 14's coefficients on entry.  ii is the last sample emitted
+// before the step and t_s the time of sample ii + 1; on return ii is the last
+// sample emitted and t_s the time of the sample after it.  Non-finite samples
+// set STATUS_NONFINITE in status.  rcont5..8 are kept without their factor h
+// (applied per sample); their position halves are h dense_k(F), the velocity
+// halves sum_j d_kj F_j.
+template <class F, class Sink>
+DRIFT_HD DRIFT_FI void dop853_dense_emit(const Dop853Job& J, const F& f, Sink& sink,
+                                         const double* y0, const double* k5, double (&Fs)[17][3],
+                                         double h, double t_step, double (&Lc0)[12],
+                                         double (&Lc1)[12], int32_t& ii, double& t_s,
+                                         int32_t& status) {
+    const double hh = h * h;
+    const double t_new = t_step + h;
+    double r1[DIM], r2[DIM], r3[DIM], r4[DIM], r5[DIM], r6[DIM], r7[DIM], r8[DIM];
+    // the three extra stages (dop853.rs:703-742)
+    rkn853_stage<14, 15>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
+    rkn853_stage<15, 16>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
+    rkn853_stage<16, 0>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
+    // rcont5..8 after the stages, one component at a time, so that each
+    // component's forces die as its eight sums consume them; every group's
+    // coefficients are fetched again per component, behind the previous
+    // group's result (scalar loads, which would not fit the scalar registers
+    // all at once)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        {
+            const Dop853Coef C = dop853_coef(J.coef, c == 0 ? Fs[16][2] : r8[2 + c]);
+            r5[c] = rkn853::dense4pos_pre(Fs, c, C) + rkn853::dense4pos_post(Fs, c, C);
+            r5[3 + c] = rkn853::dense4vel_pre(Fs, c, C) + rkn853::dense4vel_post(Fs, c, C);
+        }
+        {
+            const Dop853Coef C = dop853_coef(J.coef, r5[3 + c]);
+            r6[c] = rkn853::dense5pos_pre(Fs, c, C) + rkn853::dense5pos_post(Fs, c, C);
+            r6[3 + c] = rkn853::dense5vel_pre(Fs, c, C) + rkn853::dense5vel_post(Fs, c, C);
+        }
+        {
+            const Dop853Coef C = dop853_coef(J.coef, r6[3 + c]);
+            r7[c] = rkn853::dense6pos_pre(Fs, c, C) + rkn853::dense6pos_post(Fs, c, C);
+            r7[3 + c] = rkn853::dense6vel_pre(Fs, c, C) + rkn853::dense6vel_post(Fs, c, C);
+        }
+        {
+            const Dop853Coef C = dop853_coef(J.coef, r7[3 + c]);
+            r8[c] = rkn853::dense7pos_pre(Fs, c, C) + rkn853::dense7pos_post(Fs, c, C);
+            r8[3 + c] = rkn853::dense7vel_pre(Fs, c, C) + rkn853::dense7vel_post(Fs, c, C);
+        }
+    }
+    // rcont1..4 last: their inputs stay live anyway, the results only
+    // from here on
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        // k1 = (v0, F1), k4 (the new derivative) = (v_new, F13)
+        const double k1i = i < 3 ? y0[3 + i] : Fs[1][i - 3];
+        const double k4i = i < 3 ? k5[3 + i] : Fs[13][i - 3];
+        r1[i] = y0[i];
+        double ydiff = k5[i] - y0[i];
+        r2[i] = ydiff;
+        double bspl = h * k1i - ydiff;
+        r3[i] = bspl;
+        r4[i] = ydiff - h * k4i - bspl;
+    }
+    const double* times = J.times;
+    const int32_t last = J.nt - 1;
+    const double inv_h = rcp_fast(h);
+    do {
+        // fetch the following sample time now, so its latency hides
+        // behind this sample's polynomial and stores
+        const int32_t after = ii + 2;
+        const double t_after = after <= last ? times[after] : 0.0;
+        const double sv = (t_s - t_step) * inv_h;
+        const double s1 = 1.0 - sv;
+        const double s1h = s1 * h;
+        const double s1hh = s1h * h;  // positions carry one more h
+        double yy[DIM];
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            const double w = i < 3 ? s1hh : s1h;
+            yy[i] = r1[i] +
+                    sv * (r2[i] +
+                          s1 * (r3[i] +
+                                sv * (r4[i] + w * (r5[i] + sv * (r6[i] + s1 * (r7[i] + sv * r8[i]))))));
+        }
+        ii++;
+        if (!finite6(yy)) status |= STATUS_NONFINITE;
+        sink.emit(ii, yy);
+        t_s = t_after;
+    } while (ii < last && J.pos_neg * t_s < J.pos_neg * t_new);
+}
+
+// The trial part of one attempted DOP853 step (dop853.rs:425-619): stages 2
+// to 12 from the lane's state, the 8th-order solution k5 and the error
+// estimate, returned.  Fs[1] holds F1 on entry and Fs[2..12] the stage
+// forces on return.
 //
 // The right-hand side is (v, F(x)), so the velocity half of each stage
 // derivative is that stage's velocity, itself v0 plus h times a combination
@@ -320,16 +415,12 @@ DRIFT_HD DRIFT_FI void rkn853_stage(const Dop853Job& J, double t_step, const dou
 // coefficients of dop853_rkn.hpp (exact products of the tableau, rounded
 // once): the same method and the same values up to rounding, with half the
 // stage arithmetic and half the stage storage of the first-order form.
-template <class F, class Sink>
-DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F& f, Sink& sink) {
-    using namespace hw853;
-    const double h = J.pos_neg * tmax(std::fabs(L.h), 1e3 * uround);
+template <class F>
+DRIFT_HD DRIFT_FI double dop853_trial(const Dop853Lane& L, const Dop853Job& J, const F& f,
+                                      double h, double (&Fs)[17][3], double (&k5)[DIM]) {
     const double hh = h * h;
     const double* y0 = L.y;
     const double t_step = L.t_cur;  // start of this step
-    double Fs[17][3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) Fs[1][c] = L.F1[c];
 
     // stage coefficients, double-buffered: each stage fetches the next one's
     double Lc0[12], Lc1[12];
@@ -344,13 +435,11 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
     rkn853_stage<9, 10>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
     rkn853_stage<10, 11>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
     rkn853_stage<11, 12>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
-    const double t_old_older = L.t_old;
-    const double t_new = L.t_cur + h;
     rkn853_stage<12, 0>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
 
     // k4 = sum_j b_j k_j and the 8th-order solution k5 = y0 + h k4
     // (dop853.rs:585-595); the position half of k4 is v0 + h bpos(F)
-    double k4[DIM], k5[DIM];
+    double k4[DIM];
     {
         const Dop853Coef C = dop853_coef(J.coef, Fs[12][2]);
 #pragma unroll
@@ -392,30 +481,74 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
     double deno = err + 0.01 * err2;
     if (deno <= 0.0) deno = 1.0;
     // sqrt(1/(deno*6)) as a reciprocal square root
-    err = std::fabs(h) * err * rsqrt_d(deno * (double)DIM);
+    return std::fabs(h) * err * rsqrt_d(deno * (double)DIM);
+}
 
+// Step factors (dop853.rs:621-623, :775) as multipliers of h:
+//   accept: h / max(facc2, min(facc1, fac11 / safe)) = h min(1/facc2, shrink)
+//   reject: h / min(facc1, fac11 / safe)            = h shrink
+// with shrink = max(1/facc1, safe err^(-1/8)) and fac11 = err^(1/8).
+// err^(-1/8) is three reciprocal square roots; err = 0 (or below the normal
+// range) gives an infinite factor, err = inf a zero one, as the reference's
+// limits of pow(err, 1/8).  Returns shrink; grow_h is the accepted step's
+// candidate h min(1/facc2, shrink).
+DRIFT_HD DRIFT_FI double dop853_shrink(double err, double h, double& grow_h) {
+    using namespace hw853;
+    const double root8 = rsqrt_pos(rsqrt_pos(rsqrt_pos(err)));
+    const double grow = err > 1e-300 ? (err < INFINITY ? safe * root8 : 0.0) : INFINITY;
+    constexpr double inv_facc1 = 1.0 / facc1, inv_facc2 = 1.0 / facc2;
+    const double shrink = tmax(inv_facc1, grow);
+    grow_h = h * tmin(inv_facc2, shrink);
+    return shrink;
+}
+
+// Accepted step's bookkeeping (dop853.rs:764-773): the new state, FSAL force,
+// the next step size (clamped to hmax; no growth right after a rejection)
+// and the times.
+DRIFT_HD DRIFT_FI void dop853_commit_accept(Dop853Lane& L, const Dop853Job& J, const double* k5,
+                                            const double* F13, double h, double hnew,
+                                            double t_step, double t_new) {
+#pragma unroll
+    for (int i = 0; i < DIM; i++) L.y[i] = k5[i];
+#pragma unroll
+    for (int c = 0; c < 3; c++) L.F1[c] = F13[c];
+    hnew = std::fabs(hnew) > std::fabs(J.hmax) ? J.pos_neg * J.hmax : hnew;
+    if (L.reject != 0) hnew = J.pos_neg * tmin(std::fabs(hnew), std::fabs(h));
+    L.reject = 0;
+    L.t_old = t_step;
+    L.t_cur = t_new;
+    L.h = hnew;
+}
+
+// One attempted DOP853 step, dop853.rs:425-782, writing the samples of an
+// accepted step at once.  Returns true when the orbit has emitted its last
+// sample.
+template <class F, class Sink>
+DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F& f, Sink& sink) {
+    using namespace hw853;
+    const double h = J.pos_neg * tmax(std::fabs(L.h), 1e3 * uround);
+    const double* y0 = L.y;
+    const double t_step = L.t_cur;  // start of this step
+    const double t_old_older = L.t_old;
+    const double t_new = L.t_cur + h;
+    double Fs[17][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) Fs[1][c] = L.F1[c];
+    double k5[DIM];
+    const double err = dop853_trial(L, J, f, h, Fs, k5);
     if (err != err) {
         // The reference would shrink h to its floor and retry forever.
         abandon_orbit(L.ii, J.nt, L.status, STATUS_NONFINITE, sink);
         return true;
     }
-    // Step factors (dop853.rs:621-623, :775) as multipliers of h:
-    //   accept: h / max(facc2, min(facc1, fac11 / safe)) = h min(1/facc2, shrink)
-    //   reject: h / min(facc1, fac11 / safe)            = h shrink
-    // with shrink = max(1/facc1, safe err^(-1/8)) and fac11 = err^(1/8).
-    // err^(-1/8) is three reciprocal square roots; err = 0 (or below the
-    // normal range) gives an infinite factor, err = inf a zero one, as the
-    // reference's limits of pow(err, 1/8).
-    const double root8 = rsqrt_pos(rsqrt_pos(rsqrt_pos(err)));
-    const double grow = err > 1e-300 ? (err < INFINITY ? safe * root8 : 0.0) : INFINITY;
-    constexpr double inv_facc1 = 1.0 / facc1, inv_facc2 = 1.0 / facc2;
-    const double shrink = tmax(inv_facc1, grow);
-    double hnew = h * tmin(inv_facc2, shrink);
+    double hnew;
+    const double shrink = dop853_shrink(err, h, hnew);
 
     if (err <= 1.0) {
         L.accepted++;
         // FSAL derivative at the new point (stage 13): (v_new, F(x_new)); the
         // first dense stage's coefficients are fetched meanwhile
+        double Lc0[12], Lc1[12];
         rkn853_load<14>(dop853_coef(J.coef, k5[2]), Lc0);
         {
             double a[DIM];
@@ -423,119 +556,167 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
 #pragma unroll
             for (int c = 0; c < 3; c++) Fs[13][c] = a[3 + c];
         }
-        const double* times = J.times;
-        const int32_t last = J.nt - 1;
-        if (L.ii < last && J.pos_neg * L.t_next < J.pos_neg * t_new) {
+        if (L.ii < J.nt - 1 && J.pos_neg * L.t_next < J.pos_neg * t_new) {
             // Dense output (dop853.rs:636-763), only when a sample falls in
-            // (t_step, t_new).  rcont5..8 are kept without their factor h
-            // (applied per sample); their position halves are
-            // h dense_k(F), the velocity halves sum_j d_kj F_j.
+            // (t_step, t_new).
             L.dense++;
-            double r1[DIM], r2[DIM], r3[DIM], r4[DIM], r5[DIM], r6[DIM], r7[DIM], r8[DIM];
-            // the three extra stages (dop853.rs:703-742)
-            rkn853_stage<14, 15>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
-            rkn853_stage<15, 16>(J, t_step, y0, h, hh, Fs, Lc1, Lc0, f);
-            rkn853_stage<16, 0>(J, t_step, y0, h, hh, Fs, Lc0, Lc1, f);
-            // rcont5..8 after the stages, one coefficient group at a time (the
-            // groups' scalar loads then fit the scalar registers); the forces
-            // die as the sums consume them
-            {
-                const Dop853Coef C = dop853_coef(J.coef, Fs[16][2]);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r5[c] = rkn853::dense4pos_pre(Fs, c, C) + rkn853::dense4pos_post(Fs, c, C);
-                    r5[3 + c] = rkn853::dense4vel_pre(Fs, c, C) + rkn853::dense4vel_post(Fs, c, C);
-                }
-            }
-            {
-                const Dop853Coef C = dop853_coef(J.coef, r5);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r6[c] = rkn853::dense5pos_pre(Fs, c, C) + rkn853::dense5pos_post(Fs, c, C);
-                    r6[3 + c] = rkn853::dense5vel_pre(Fs, c, C) + rkn853::dense5vel_post(Fs, c, C);
-                }
-            }
-            {
-                const Dop853Coef C = dop853_coef(J.coef, r6);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r7[c] = rkn853::dense6pos_pre(Fs, c, C) + rkn853::dense6pos_post(Fs, c, C);
-                    r7[3 + c] = rkn853::dense6vel_pre(Fs, c, C) + rkn853::dense6vel_post(Fs, c, C);
-                }
-            }
-            {
-                const Dop853Coef C = dop853_coef(J.coef, r7);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    r8[c] = rkn853::dense7pos_pre(Fs, c, C) + rkn853::dense7pos_post(Fs, c, C);
-                    r8[3 + c] = rkn853::dense7vel_pre(Fs, c, C) + rkn853::dense7vel_post(Fs, c, C);
-                }
-            }
-            // rcont1..4 last: their inputs stay live anyway, the results only
-            // from here on
-#pragma unroll
-            for (int i = 0; i < DIM; i++) {
-                // k1 = (v0, F1), k4 (the new derivative) = (v_new, F13)
-                const double k1i = i < 3 ? y0[3 + i] : Fs[1][i - 3];
-                const double k4i = i < 3 ? k5[3 + i] : Fs[13][i - 3];
-                r1[i] = y0[i];
-                double ydiff = k5[i] - y0[i];
-                r2[i] = ydiff;
-                double bspl = h * k1i - ydiff;
-                r3[i] = bspl;
-                r4[i] = ydiff - h * k4i - bspl;
-            }
-            const double inv_h = rcp_fast(h);
             double t_s = L.t_next;
-            do {
-                // fetch the following sample time now, so its latency hides
-                // behind this sample's polynomial and stores
-                const int32_t after = L.ii + 2;
-                const double t_after = after <= last ? times[after] : 0.0;
-                const double sv = (t_s - t_step) * inv_h;
-                const double s1 = 1.0 - sv;
-                const double s1h = s1 * h;
-                const double s1hh = s1h * h;  // positions carry one more h
-                double yy[DIM];
-#pragma unroll
-                for (int i = 0; i < DIM; i++) {
-                    const double w = i < 3 ? s1hh : s1h;
-                    yy[i] = r1[i] +
-                            sv * (r2[i] +
-                                  s1 * (r3[i] +
-                                        sv * (r4[i] + w * (r5[i] + sv * (r6[i] + s1 * (r7[i] + sv * r8[i]))))));
-                }
-                L.ii++;
-                if (!finite6(yy)) L.status |= STATUS_NONFINITE;
-                sink.emit(L.ii, yy);
-                t_s = t_after;
-            } while (L.ii < last && J.pos_neg * t_s < J.pos_neg * t_new);
+            dop853_dense_emit(J, f, sink, y0, k5, Fs, h, t_step, Lc0, Lc1, L.ii, t_s, L.status);
             L.t_next = t_s;
         }
-#pragma unroll
-        for (int i = 0; i < DIM; i++) L.y[i] = k5[i];
-#pragma unroll
-        for (int c = 0; c < 3; c++) L.F1[c] = Fs[13][c];
-        hnew = std::fabs(hnew) > std::fabs(J.hmax) ? J.pos_neg * J.hmax : hnew;
-        if (L.reject != 0) hnew = J.pos_neg * tmin(std::fabs(hnew), std::fabs(h));
-        L.reject = 0;
-        L.t_old = t_step;
-        L.t_cur = t_new;
+        dop853_commit_accept(L, J, k5, Fs[13], h, hnew, t_step, t_new);
     } else {
         L.rejected++;
-        hnew = h * shrink;
         L.reject = 1;
         // t_current = t_old; t_old = t_old_older (dop853.rs:777-778)
         L.t_cur = t_step;
         L.t_old = t_old_older;
+        L.h = h * shrink;
     }
-    L.h = hnew;
     if (L.ii >= J.nt - 1) return true;
     if (L.accepted + L.rejected >= J.max_attempts) {
         abandon_orbit(L.ii, J.nt, L.status, STATUS_STEP_LIMIT, sink);
         return true;
     }
     return false;
+}
+
+// The inputs of one accepted step's dense output, as a lane hands them to
+// another lane of its wavefront: the step's start y0 and end k5, the stage
+// forces F1 and F6..F13, h and the step's start time, the orbit's output
+// index, the last sample emitted before the step and the lane that owns the
+// orbit.
+struct Dop853Dense {
+    double y0[DIM], k5[DIM];
+    double F[9][3];  // F1, F6, F7, ..., F13
+    double h, t_step;
+    int64_t idx;
+    int32_t ii, owner;
+};
+
+// stage index of Dop853Dense::F[q]
+DRIFT_HD DRIFT_FI constexpr int dense_force_stage(int q) { return q == 0 ? 1 : q + 5; }
+
+// The same attempted step with its dense output handed over instead of
+// written: an accepted step with samples fills a Dop853Dense (owner and idx
+// are left to the caller), the lane's sample index and next sample time move
+// past the samples the step covers, and round(own, req, completes) runs
+// before the step is committed -- at a point every lane of the caller
+// reaches, with own set where this lane's step left req -- so that whoever
+// runs the request (dop853_run_dense) writes the samples.  completes is set
+// where this attempt ends the orbit (its last sample, a non-finite error
+// estimate or the attempt limit).  Counts, samples and the returned
+// completion are those of dop853_attempt.
+template <class F, class Sink, class Round>
+DRIFT_HD DRIFT_FI bool dop853_attempt_deferred(Dop853Lane& L, const Dop853Job& J, const F& f,
+                                               Sink& sink, Round&& round) {
+    using namespace hw853;
+    const double h = J.pos_neg * tmax(std::fabs(L.h), 1e3 * uround);
+    const double t_step = L.t_cur;
+    const double t_new = L.t_cur + h;
+    const int32_t last = J.nt - 1;
+    double Fs[17][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) Fs[1][c] = L.F1[c];
+    double k5[DIM];
+    const double err = dop853_trial(L, J, f, h, Fs, k5);
+    const bool nonfinite = err != err;
+    double hnew;
+    const double shrink = dop853_shrink(err, h, hnew);
+    const bool accept = err <= 1.0;  // false for a NaN
+    Dop853Dense req;
+    bool own = false;
+    if (accept) {
+        L.accepted++;
+        const double* times = J.times;
+        own = L.ii < last && J.pos_neg * L.t_next < J.pos_neg * t_new;
+        // the times of the two samples after the first, fetched before the
+        // FSAL force, which hides their latency
+        double t_pre[2] = {0.0, 0.0};
+        if (own) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) t_pre[q] = L.ii + 2 + q <= last ? times[L.ii + 2 + q] : 0.0;
+        }
+        {
+            double a[DIM];
+            f(t_new, k5, a);
+#pragma unroll
+            for (int c = 0; c < 3; c++) Fs[13][c] = a[3 + c];
+        }
+        if (own) {
+            L.dense++;
+#pragma unroll
+            for (int i = 0; i < DIM; i++) {
+                req.y0[i] = L.y[i];
+                req.k5[i] = k5[i];
+            }
+#pragma unroll
+            for (int q = 0; q < 9; q++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) req.F[q][c] = Fs[dense_force_stage(q)][c];
+            req.h = h;
+            req.t_step = t_step;
+            req.ii = L.ii;
+            // the samples the step covers, as dop853_dense_emit's loop walks
+            // them: t_s is the time of sample ii + 1 after each
+            int32_t ii = L.ii + 1;
+            double t_s = t_pre[0];
+            if (ii < last && J.pos_neg * t_s < J.pos_neg * t_new) {
+                ii++;
+                t_s = t_pre[1];
+                while (ii < last && J.pos_neg * t_s < J.pos_neg * t_new) {
+                    const int32_t after = ii + 2;
+                    t_s = after <= last ? times[after] : 0.0;
+                    ii++;
+                }
+            }
+            L.ii = ii;
+            L.t_next = t_s;
+        }
+    } else if (!nonfinite) {
+        L.rejected++;
+    }
+    const bool completes = nonfinite || L.ii >= last || L.accepted + L.rejected >= J.max_attempts;
+    round(own, req, completes);
+    if (nonfinite) {
+        // The reference would shrink h to its floor and retry forever.
+        abandon_orbit(L.ii, J.nt, L.status, STATUS_NONFINITE, sink);
+        return true;
+    }
+    if (accept) {
+        dop853_commit_accept(L, J, k5, Fs[13], h, hnew, t_step, t_new);
+    } else {
+        // t_current and t_old keep their values (dop853.rs:777-778)
+        L.reject = 1;
+        L.h = h * shrink;
+    }
+    if (L.ii >= last) return true;
+    if (L.accepted + L.rejected >= J.max_attempts) {
+        abandon_orbit(L.ii, J.nt, L.status, STATUS_STEP_LIMIT, sink);
+        return true;
+    }
+    return false;
+}
+
+// Runs a handed-over dense output on the calling lane: the request's stage
+// forces go back into their stage slots, and the samples go to the request's
+// orbit through sink (whose orbit index the caller has set to req.idx).
+// Returns the STATUS_NONFINITE bit if a sample was not finite.
+template <class F, class Sink>
+DRIFT_HD DRIFT_FI int32_t dop853_run_dense(const Dop853Dense& req, const Dop853Job& J, const F& f,
+                                           Sink& sink) {
+    double Fs[17][3];
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) Fs[dense_force_stage(q)][c] = req.F[q][c];
+    double Lc0[12], Lc1[12];
+    rkn853_load<14>(dop853_coef(J.coef, req.h), Lc0);
+    int32_t ii = req.ii;
+    double t_s = J.times[ii + 1];
+    int32_t status = 0;
+    dop853_dense_emit(J, f, sink, req.y0, req.k5, Fs, req.h, req.t_step, Lc0, Lc1, ii, t_s, status);
+    return status;
 }
 
 // ---------------------------------------------------------------------------

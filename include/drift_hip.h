@@ -114,7 +114,13 @@ typedef struct drift_orbit_stats {
 /* Launch tuning knobs for the device-resident entry points. */
 typedef struct drift_launch_options {
     int32_t precision;  /* drift_precision */
-    int32_t block_size; /* threads per block; 0 = library default */
+    int32_t block_size; /* threads per block: 64..256 in steps of 64, or 512 for DOP853
+                           with a static potential (dense output packed across each
+                           wavefront's lanes: accepted steps hand their dense-output
+                           inputs to an LDS stash, and a wavefront runs them once most
+                           of its lanes have one; bit-identical results); 0 = library
+                           default: 512 for persistent DOP853 launches with direct output
+                           and a static potential, else 256 */
     int32_t schedule;   /* drift_schedule */
     uint32_t max_attempts; /* per-orbit attempted-step limit; 0 = default (2^22).
                               The reference has none: a DOP853 orbit whose error

@@ -2,6 +2,7 @@
 // (astrodrift_amd/csrc/integrators.hpp with the forces of forces.hpp), for CPU
 // tests of their arithmetic against the oracle.  Test infrastructure: the library itself
 // only runs these functions inside its gfx950 kernels.
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -36,6 +37,80 @@ void run(const F& f, int64_t n, const double* y0, const double* t, int nt, doubl
         stats[4 * i + 1] = L.rejected;
         stats[4 * i + 2] = L.dense;
         stats[4 * i + 3] = L.status;
+    }
+}
+
+// The deferred form of the same integration (dop853_attempt_deferred):
+// groups of `lanes` orbits step together; an accepted step with samples
+// leaves a Dop853Dense request, which
+// waits in a stash of `slots` requests until `run_at` are pending (or an orbit
+// of the group completes), as the packed GPU kernels do, and is then run by
+// dop853_run_dense.  The outputs must equal run()'s bit for bit.
+template <class F>
+void run_deferred(const F& f, int64_t n, const double* y0, const double* t, int nt, double rtol,
+                  double atol, double* out, int64_t* stats, int lanes, int slots, int run_at) {
+    const Dop853Job J = make_dop853_job(t, nt, rtol, atol, 1u << 22, t[0], t[nt - 1]);
+    for (int64_t g0 = 0; g0 < n; g0 += lanes) {
+        const int m = (int)std::min<int64_t>(lanes, n - g0);
+        std::vector<Dop853Lane> L(m);
+        std::vector<bool> live(m, true);
+        std::vector<int32_t> nf(m, 0);
+        std::vector<Dop853Dense> stash;
+        for (int l = 0; l < m; l++) {
+            HostSink sink{out, n, g0 + l};
+            dop853_init(L[l], y0 + DIM * (g0 + l), t[0], J, f, sink);
+        }
+        auto run_req = [&](const Dop853Dense& r) {
+            HostSink sink{out, n, r.idx};
+            nf[r.owner] |= dop853_run_dense(r, J, f, sink);
+        };
+        for (;;) {
+            std::vector<Dop853Dense> own;
+            std::vector<int> fin;
+            // the lanes' attempts; their requests run after all of them (a
+            // request writes only its own orbit's samples, so when it runs
+            // against the owner's commit does not matter)
+            for (int l = 0; l < m; l++) {
+                if (!live[l]) continue;
+                HostSink sink{out, n, g0 + l};
+                const bool done = dop853_attempt_deferred(
+                    L[l], J, f, sink, [&](bool has, Dop853Dense& r, bool) {
+                        if (!has) return;
+                        r.idx = g0 + l;
+                        r.owner = l;
+                        own.push_back(r);
+                    });
+                if (done) fin.push_back(l);
+            }
+            const int k = (int)own.size(), p = (int)stash.size();
+            const bool drain = !fin.empty();
+            if (!drain && k + p < run_at && k + p <= slots) {
+                stash.insert(stash.end(), own.begin(), own.end());
+            } else if (k + p > 0) {
+                for (const auto& r : own) run_req(r);
+                // lanes without a request of their own take stashed ones, from the top
+                int take = std::min(p, lanes - k);
+                while (take-- > 0) {
+                    run_req(stash.back());
+                    stash.pop_back();
+                }
+                if (drain)
+                    while (!stash.empty()) {
+                        run_req(stash.back());
+                        stash.pop_back();
+                    }
+            }
+            for (int l : fin) {
+                stats[4 * (g0 + l)] = L[l].accepted;
+                stats[4 * (g0 + l) + 1] = L[l].rejected;
+                stats[4 * (g0 + l) + 2] = L[l].dense;
+                stats[4 * (g0 + l) + 3] = L[l].status | nf[l];
+                live[l] = false;
+            }
+            bool any = false;
+            for (int l = 0; l < m; l++) any = any || live[l];
+            if (!any) break;
+        }
     }
 }
 
@@ -83,17 +158,16 @@ extern "C" int host_dopr54(const drift_potential* p, int64_t n, const double* y0
 }
 
 // potential: NULL = the unit galpy Kepler force; otherwise a drift_potential
-// (Kepler, Plummer or Milky-Way).  Linear tolerances.
-extern "C" int host_dop853(const drift_potential* p, int64_t n, const double* y0, const double* t,
-                           int nt, double rtol, double atol, double* out, int64_t* stats) {
+// (Kepler, Plummer or Milky-Way).  Linear tolerances.  lanes = 0 runs the
+// inline step machine, lanes > 0 the deferred one (run_deferred).
+template <class Run>
+int with_force(const drift_potential* p, const Run& go) {
     if (p == nullptr) {
-        run(GalpyKeplerForce<double>{}, n, y0, t, nt, rtol, atol, out, stats);
-        return 0;
-    }
-    if (p->kind == DRIFT_POTENTIAL_KEPLER) {
-        run(KeplerForce<double>{p->amp}, n, y0, t, nt, rtol, atol, out, stats);
+        go(GalpyKeplerForce<double>{});
+    } else if (p->kind == DRIFT_POTENTIAL_KEPLER) {
+        go(KeplerForce<double>{p->amp});
     } else if (p->kind == DRIFT_POTENTIAL_PLUMMER) {
-        run(PlummerForce<double>{p->amp, p->b * p->b}, n, y0, t, nt, rtol, atol, out, stats);
+        go(PlummerForce<double>{p->amp, p->b * p->b});
     } else if (p->kind == DRIFT_POTENTIAL_MW) {
         PotParams pp;
         build_mw_params(*p, pp);
@@ -101,10 +175,22 @@ extern "C" int host_dop853(const drift_potential* p, int64_t n, const double* y0
         build_mw_table(*p, tab.data());
         build_mw_far_table(*p, far.data());
         pp.far_table = far.data();
-        run(MwForce<double>{&pp, tab.data(), pp.disk_amp, pp.disk_a, pp.disk_b2}, n, y0, t, nt,
-            rtol, atol, out, stats);
+        go(MwForce<double>{&pp, tab.data(), pp.disk_amp, pp.disk_a, pp.disk_b2});
     } else {
         return 1;
     }
     return 0;
+}
+
+extern "C" int host_dop853(const drift_potential* p, int64_t n, const double* y0, const double* t,
+                           int nt, double rtol, double atol, double* out, int64_t* stats) {
+    return with_force(p, [&](const auto& f) { run(f, n, y0, t, nt, rtol, atol, out, stats); });
+}
+
+extern "C" int host_dop853_deferred(const drift_potential* p, int64_t n, const double* y0,
+                                    const double* t, int nt, double rtol, double atol, double* out,
+                                    int64_t* stats, int lanes, int slots, int run_at) {
+    return with_force(p, [&](const auto& f) {
+        run_deferred(f, n, y0, t, nt, rtol, atol, out, stats, lanes, slots, run_at);
+    });
 }

@@ -123,6 +123,8 @@ def host_lib(tmp_path_factory):
     lib.host_dop853.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
                                 ctypes.c_void_p]
+    lib.host_dop853_deferred.restype = ctypes.c_int
+    lib.host_dop853_deferred.argtypes = lib.host_dop853.argtypes + [ctypes.c_int] * 3
     return lib
 
 
@@ -134,6 +136,18 @@ def _host(lib, spec, y0, times, rtol, atol):
     pot = spec.to_c() if spec is not None else None
     assert lib.host_dop853(ctypes.byref(pot) if pot is not None else None, n, y0.ctypes.data,
                            times.ctypes.data, nt, rtol, atol, out.ctypes.data, st.ctypes.data) == 0
+    return out, st
+
+
+def _host_deferred(lib, spec, y0, times, rtol, atol, lanes, slots, run_at):
+    y0 = np.ascontiguousarray(y0, dtype=np.float64)
+    n, nt = y0.shape[0], times.shape[0]
+    out = np.full((nt, n, 6), np.nan)
+    st = np.zeros((n, 4), np.int64)
+    pot = spec.to_c() if spec is not None else None
+    assert lib.host_dop853_deferred(ctypes.byref(pot) if pot is not None else None, n,
+                                    y0.ctypes.data, times.ctypes.data, nt, rtol, atol,
+                                    out.ctypes.data, st.ctypes.data, lanes, slots, run_at) == 0
     return out, st
 
 
@@ -170,6 +184,30 @@ def test_host_step_machine_matches_oracle_point_masses(host_lib, pot):
     print(f"{pot}: pos {ep:.2e} vel {ev:.2e}")
     assert ep < 1e-8 and ev < 1e-8
     assert abs(int(st[:, 0].sum()) - int(rst["accepted"].sum())) <= 1e-3 * rst["accepted"].sum()
+
+
+@pytest.mark.parametrize("pot", ["mw", "galpy", "plummer"])
+@pytest.mark.parametrize("lanes,slots,run_at", [(64, 36, 48), (16, 6, 12), (8, 0, 1)])
+def test_deferred_dense_output_equals_inline(host_lib, pot, lanes, slots, run_at):
+    """The packed kernels' split of DOP853's dense output: an accepted step
+    leaves its dense-output inputs (dop853_attempt_impl<true>), which another
+    lane runs later (dop853_run_dense).  Stashed, drained on completion or run
+    at once, every sample and every step statistic equals the inline step
+    machine's bit for bit, on grids with one and with several samples per
+    step."""
+    if pot == "mw":
+        w = workloads.mw_disk()
+        y0 = w.state0[np.arange(0, w.n, w.n // 96)]
+        spec, times, tols = w.potential, w.times, [w.rtol, 1e-7]
+    else:
+        k = workloads.kepler_disk(n=96)
+        y0, times, tols = k.state0, k.times, [k.rtol, 1e-6]
+        spec = None if pot == "galpy" else _native.PotentialSpec.plummer(1.0, 0.1)
+    for tol in tols:
+        ref, rst = _host(host_lib, spec, y0, times, tol, tol)
+        out, st = _host_deferred(host_lib, spec, y0, times, tol, tol, lanes, slots, run_at)
+        assert np.array_equal(st, rst)
+        assert np.array_equal(out, ref), np.nanmax(np.abs(out - ref))
 
 
 def test_dopr54_header_is_current():
