@@ -327,15 +327,17 @@ __global__ void __launch_bounds__(kDefaultBlock) dopr54_persistent_kernel(PlainA
 // lanes with a request from this step run their own, the others take requests
 // from the stash.  Samples are written by whichever lane runs the request, to
 // the request's orbit, with the same arithmetic, so outputs are bit-identical
-// to the inline branch.  A lane whose orbit completes makes its wavefront
-// drain the stash before it writes the orbit's statistics, so the
-// non-finite flag that a lane running one of its requests may set (in an LDS
-// word per lane) is in.  One 512-thread block per CU shares one copy of the
-// Milky-Way table; the rest of the LDS holds the eight stashes.
+// to the inline branch.  An orbit may finish with requests still stashed;
+// its statistics record is written then, its status OR-ed in (the launch
+// zeroes the status words first), and a lane that runs one of its requests
+// and writes a non-finite sample ORs the flag in as well.  A wavefront whose
+// lanes have all run out of orbits runs what is left in its stash before it
+// leaves.  One 512-thread block per CU shares one copy of the Milky-Way
+// table; the rest of the LDS holds the eight stashes.
 // ---------------------------------------------------------------------------
 constexpr int kPackedBlock = 512;
 constexpr int kPackedWaves = kPackedBlock / 64;
-constexpr int kDenseWords = 43;  // words of a stashed Dop853Dense
+constexpr int kDenseWords = kDenseRecordWords;  // words of a stashed Dop853Dense
 constexpr int kDenseRunAt = 48;  // pending requests from which a wavefront runs them
 constexpr int kLdsBytes = 163840;
 
@@ -344,13 +346,11 @@ constexpr int table_words() {
     return kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1;
 }
 
-// Stash slots per wavefront: what the LDS holds beside the table and the
-// flags, at most 56.
+// Stash slots per wavefront: what the LDS holds beside the table, at most 56.
 template <int KIND, class FT>
 constexpr int packed_slots() {
-    constexpr int free_bytes =
-        kLdsBytes - table_words<KIND, FT>() * (int)sizeof(FT) - kPackedWaves * 64 * 4 - 256;
-    constexpr int d = free_bytes / (kPackedWaves * (kDenseWords + 1) * 8);
+    constexpr int free_bytes = kLdsBytes - table_words<KIND, FT>() * (int)sizeof(FT) - 256;
+    constexpr int d = free_bytes / (kPackedWaves * kDenseWords * 8);
     return d < 56 ? d : 56;
 }
 
@@ -362,48 +362,8 @@ __device__ __forceinline__ void wave_lds_sync() { __asm__ __volatile__("" ::: "m
 
 // A stash is field-major in 16-byte words: words 2j and 2j + 1 of slot s at
 // double2 index j * D + s, so the lanes of one access touch consecutive
-// 16-byte words.  kDenseWords is odd; the last pair's second word is unused.
-constexpr int kDensePairs = (kDenseWords + 1) / 2;
-
-struct DenseWords {
-    double w[2 * kDensePairs];
-};
-
-__device__ __forceinline__ void dense_pack(const Dop853Dense& r, DenseWords& x) {
-    int w = 0;
-#pragma unroll
-    for (int i = 0; i < DIM; i++) x.w[w++] = r.y0[i];
-#pragma unroll
-    for (int i = 0; i < DIM; i++) x.w[w++] = r.k5[i];
-#pragma unroll
-    for (int q = 0; q < 9; q++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) x.w[w++] = r.F[q][c];
-    x.w[w++] = r.h;
-    x.w[w++] = r.t_step;
-    x.w[w++] = __longlong_as_double((long long)r.idx);
-    x.w[w++] = __longlong_as_double(
-        (long long)(((uint64_t)(uint32_t)r.owner << 32) | (uint32_t)r.ii));
-    x.w[w++] = 0.0;
-}
-
-__device__ __forceinline__ void dense_unpack(const DenseWords& x, Dop853Dense& r) {
-    int w = 0;
-#pragma unroll
-    for (int i = 0; i < DIM; i++) r.y0[i] = x.w[w++];
-#pragma unroll
-    for (int i = 0; i < DIM; i++) r.k5[i] = x.w[w++];
-#pragma unroll
-    for (int q = 0; q < 9; q++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) r.F[q][c] = x.w[w++];
-    r.h = x.w[w++];
-    r.t_step = x.w[w++];
-    r.idx = (int64_t)__double_as_longlong(x.w[w++]);
-    const uint64_t packed = (uint64_t)__double_as_longlong(x.w[w++]);
-    r.ii = (int32_t)(uint32_t)packed;
-    r.owner = (int32_t)(packed >> 32);
-}
+// 16-byte words (the record's layout: dense_pack).
+constexpr int kDensePairs = kDenseWords / 2;
 
 template <int D>
 __device__ __forceinline__ void stash_put(double2* W, int s, const Dop853Dense& r) {
@@ -427,50 +387,54 @@ __device__ __forceinline__ void stash_get(const double2* W, int s, Dop853Dense& 
 
 // One wavefront's dense output after an attempted step.  own: this lane's
 // step left request R.  p: the wavefront's pending requests (slots 0..p-1 of
-// its stash W).  drain: run every pending request now.  Otherwise the
-// requests are stashed while they fit and fall short of kDenseRunAt; when
-// they run, the lanes without a request of their own take stashed ones from
-// the top.
+// its stash W).  Unless force is set, the requests are stashed while they fit
+// and fall short of kDenseRunAt.  Otherwise one pass runs: the lanes with a
+// request of their own run it, the other lanes take stashed ones from the top
+// (so a pass with force set and no own requests runs up to 64 stashed ones).
+// Called by every lane of the wavefront.
 template <int D, class F>
-__device__ __forceinline__ void dense_round(double2* W, int32_t* nf, int& p, bool own,
-                                            Dop853Dense& R, bool drain, const Dop853Job& J,
-                                            const F& f, OutSink sink) {
+__device__ __forceinline__ void dense_round(double2* W, int& p, bool own, Dop853Dense& R,
+                                            bool force, const Dop853Job& J, const F& f,
+                                            OutSink sink, drift_orbit_stats* stats) {
     const uint64_t ownm = __ballot(own);
     const int k = __popcll(ownm);
     if (k + p == 0) return;
     const uint64_t below = (1ull << __lane_id()) - 1ull;
-    if (!drain && k + p < kDenseRunAt && k + p <= D) {
+    if (!force && k + p < kDenseRunAt && k + p <= D) {
         if (own) stash_put<D>(W, p + __popcll(ownm & below), R);
         p += k;
         wave_lds_sync();
         return;
     }
-    bool mine = own;
-    uint64_t busy = ownm;
-    const uint64_t active = __ballot(true);
-    for (;;) {
-        const uint64_t free_lanes = active & ~busy;
-        const int take = min(p, __popcll(free_lanes));
-        bool popped = false;
-        if (!mine) {
-            const int r = __popcll(free_lanes & below);
-            if (r < take) {
-                stash_get<D>(W, p - 1 - r, R);
-                popped = true;
-            }
+    const uint64_t free_lanes = __ballot(true) & ~ownm;
+    const int take = min(p, __popcll(free_lanes));
+    bool popped = false;
+    if (!own) {
+        const int r = __popcll(free_lanes & below);
+        if (r < take) {
+            stash_get<D>(W, p - 1 - r, R);
+            popped = true;
         }
-        p -= take;
-        wave_lds_sync();
-        if (mine || popped) {
-            sink.idx = R.idx;
-            const int32_t st = dop853_run_dense(R, J, f, sink);
-            if (st) atomicOr(&nf[R.owner], st);
-        }
-        if (!drain || p == 0) break;
-        mine = false;
-        busy = 0;
     }
+    p -= take;
     wave_lds_sync();
+    if (own || popped) {
+        sink.idx = R.idx;
+        const int32_t st = dop853_run_dense(R, J, f, sink);
+        if (st && stats) atomicOr(&stats[R.idx].status, st);
+    }
+}
+
+// An orbit's statistics record: the counts, and its status OR-ed into the
+// zeroed status word (a lane running one of the orbit's stashed requests may
+// OR in a non-finite flag before or after).
+__device__ __forceinline__ void write_packed_stats(drift_orbit_stats* stats, int64_t i,
+                                                   const Dop853Lane& L) {
+    if (!stats) return;
+    stats[i].accepted = L.accepted;
+    stats[i].rejected = L.rejected;
+    stats[i].dense_steps = L.dense;
+    if (L.status) atomicOr(&stats[i].status, L.status);
 }
 
 template <int KIND, class FT>
@@ -479,11 +443,8 @@ dop853_plain_packed_kernel(PlainArgs a) {
     constexpr int D = packed_slots<KIND, FT>();
     __shared__ FT lds[table_words<KIND, FT>()];
     __shared__ double2 stash[kPackedWaves][kDensePairs * D];
-    __shared__ int32_t nf[kPackedWaves][64];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
-    const int wave = (int)(threadIdx.x >> 6), lane = __lane_id();
-    nf[wave][lane] = 0;
-    wave_lds_sync();
+    const int wave = (int)(threadIdx.x >> 6);
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const auto f = KindForce<KIND, FT>::make(a.pot, table);
     const double t0 = a.times[0];
@@ -501,6 +462,10 @@ dop853_plain_packed_kernel(PlainArgs a) {
     }
     int p = 0;
     for (;;) {
+        // every orbit ends (at the latest after max_attempts); then the
+        // wavefront runs what is left in its stash and leaves
+        const bool any_live = __ballot(live) != 0;
+        if (!any_live && p == 0) break;
         bool fin = false, own = false;
         Dop853Dense R;
         if (live)
@@ -509,21 +474,11 @@ dop853_plain_packed_kernel(PlainArgs a) {
                 if (has) R = req;
             });
         R.idx = sink.idx;
-        R.owner = lane;
-        // every lane takes part: lanes without an orbit run stashed requests
-        dense_round<D>(stash[wave], nf[wave], p, own, R, __ballot(fin) != 0, J, f, sink);
+        dense_round<D>(stash[wave], p, own, R, !any_live, J, f, sink, a.stats);
         if (fin) {
-            if (a.stats) {
-                drift_orbit_stats st;
-                st.accepted = L.accepted;
-                st.rejected = L.rejected;
-                st.dense_steps = L.dense;
-                st.status = L.status | nf[wave][lane];
-                a.stats[sink.idx] = st;
-            }
+            write_packed_stats(a.stats, sink.idx, L);
             live = false;
         }
-        if (__ballot(live) == 0) break;
     }
 }
 
@@ -533,11 +488,8 @@ dop853_persistent_packed_kernel(PlainArgs a) {
     constexpr int D = packed_slots<KIND, FT>();
     __shared__ FT lds[table_words<KIND, FT>()];
     __shared__ double2 stash[kPackedWaves][kDensePairs * D];
-    __shared__ int32_t nf[kPackedWaves][64];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
-    const int wave = (int)(threadIdx.x >> 6), lane = __lane_id();
-    nf[wave][lane] = 0;
-    wave_lds_sync();
+    const int wave = (int)(threadIdx.x >> 6);
     const auto f = KindForce<KIND, FT>::make(a.pot, table);
     const double t0 = a.times[0];
     const Dop853Job J = make_dop853_job(a.times, a.nt, a.rtol, a.atol, a.max_attempts, t0,
@@ -557,9 +509,10 @@ dop853_persistent_packed_kernel(PlainArgs a) {
             need = false;
         }
         // every orbit ends (at the latest after max_attempts), so every lane
-        // reaches an empty queue and the wavefront leaves once none is live;
-        // an orbit's last attempt drains its wavefront's stash
-        if (__ballot(live) == 0) break;
+        // reaches an empty queue; then the wavefront runs what is left in its
+        // stash and leaves
+        const bool any_live = __ballot(live) != 0;
+        if (!any_live && p == 0) break;
         bool fin = false, own = false;
         Dop853Dense R;
         if (live)
@@ -568,22 +521,19 @@ dop853_persistent_packed_kernel(PlainArgs a) {
                 if (has) R = req;
             });
         R.idx = sink.idx;
-        R.owner = lane;
-        // every lane takes part: lanes without an orbit run stashed requests
-        dense_round<D>(stash[wave], nf[wave], p, own, R, __ballot(fin) != 0, J, f, sink);
+        dense_round<D>(stash[wave], p, own, R, !any_live, J, f, sink, a.stats);
         if (fin) {
-            if (a.stats) {
-                drift_orbit_stats st;
-                st.accepted = L.accepted;
-                st.rejected = L.rejected;
-                st.dense_steps = L.dense;
-                st.status = L.status | nf[wave][lane];
-                a.stats[sink.idx] = st;
-            }
-            nf[wave][lane] = 0;
+            write_packed_stats(a.stats, sink.idx, L);
             need = true;
         }
     }
+}
+
+// Zeroes the status words of n statistics records (the packed kernels OR
+// into them).
+__global__ void zero_status_kernel(drift_orbit_stats* stats, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) stats[i].status = 0;
 }
 
 __global__ void reset_queue_kernel(unsigned long long* queue) { *queue = 0ull; }
@@ -1290,6 +1240,11 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
         a.out = static_cast<double*>(s->staging.ptr);
         a.staged = 1;
     }
+    if (packed && a.stats) {
+        hipLaunchKernelGGL(zero_status_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           a.stats, (int64_t)n);
+        DRIFT_HIP_TRY(hipGetLastError());
+    }
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, st, a);
     DRIFT_HIP_TRY(hipGetLastError());
     s->last_grid = (int32_t)blocks;
@@ -1598,6 +1553,11 @@ int noalloc_dop853(const double* d_state0, const double* d_times, double* d_out,
             DRIFT_OK)
             return rc;
         a.order = slots;
+    }
+    if (packed && a.stats) {
+        hipLaunchKernelGGL(zero_status_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           a.stats, (int64_t)n);
+        DRIFT_HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(block), 0, st, a);
     DRIFT_HIP_TRY(hipGetLastError());

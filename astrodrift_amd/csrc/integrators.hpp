@@ -188,7 +188,10 @@ struct Dop853Job {
 struct Dop853Lane {
     double y[DIM];   // (x, v)
     double F1[3];    // force at y: the acceleration half of the first stage derivative
-    double h, t_cur, t_old;
+    // the reference's t_old (the last accepted step's start) only feeds its
+    // dense output, which here takes the step's start directly, so the lane
+    // does not keep it
+    double h, t_cur;
     double t_next;  // times[ii + 1] while ii < nt - 1 (the next sample to reach)
     int32_t ii;  // index of the last emitted sample
     int32_t reject;
@@ -248,7 +251,6 @@ DRIFT_HD DRIFT_FI void dop853_init(Dop853Lane& L, const double* y0, double t0, c
     double h1 = std::sqrt(std::sqrt(std::sqrt(0.01 / der12)));  // (0.01/der12)^(1/8)
     L.h = sign_of(tmin(100.0 * std::fabs(h), tmin(std::fabs(h1), std::fabs(J.hmax))), J.pos_neg);
     L.t_cur = t0;
-    L.t_old = t0;
     L.ii = 0;
     L.t_next = J.nt > 1 ? J.times[1] : 0.0;
     L.reject = 0;
@@ -507,7 +509,7 @@ DRIFT_HD DRIFT_FI double dop853_shrink(double err, double h, double& grow_h) {
 // and the times.
 DRIFT_HD DRIFT_FI void dop853_commit_accept(Dop853Lane& L, const Dop853Job& J, const double* k5,
                                             const double* F13, double h, double hnew,
-                                            double t_step, double t_new) {
+                                            double t_new) {
 #pragma unroll
     for (int i = 0; i < DIM; i++) L.y[i] = k5[i];
 #pragma unroll
@@ -515,7 +517,6 @@ DRIFT_HD DRIFT_FI void dop853_commit_accept(Dop853Lane& L, const Dop853Job& J, c
     hnew = std::fabs(hnew) > std::fabs(J.hmax) ? J.pos_neg * J.hmax : hnew;
     if (L.reject != 0) hnew = J.pos_neg * tmin(std::fabs(hnew), std::fabs(h));
     L.reject = 0;
-    L.t_old = t_step;
     L.t_cur = t_new;
     L.h = hnew;
 }
@@ -529,7 +530,6 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
     const double h = J.pos_neg * tmax(std::fabs(L.h), 1e3 * uround);
     const double* y0 = L.y;
     const double t_step = L.t_cur;  // start of this step
-    const double t_old_older = L.t_old;
     const double t_new = L.t_cur + h;
     double Fs[17][3];
 #pragma unroll
@@ -564,13 +564,12 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
             dop853_dense_emit(J, f, sink, y0, k5, Fs, h, t_step, Lc0, Lc1, L.ii, t_s, L.status);
             L.t_next = t_s;
         }
-        dop853_commit_accept(L, J, k5, Fs[13], h, hnew, t_step, t_new);
+        dop853_commit_accept(L, J, k5, Fs[13], h, hnew, t_new);
     } else {
         L.rejected++;
+        // t_current goes back to the step's start, which it still holds
+        // (dop853.rs:777-778)
         L.reject = 1;
-        // t_current = t_old; t_old = t_old_older (dop853.rs:777-778)
-        L.t_cur = t_step;
-        L.t_old = t_old_older;
         L.h = h * shrink;
     }
     if (L.ii >= J.nt - 1) return true;
@@ -584,22 +583,66 @@ DRIFT_HD DRIFT_FI bool dop853_attempt(Dop853Lane& L, const Dop853Job& J, const F
 // The inputs of one accepted step's dense output, as a lane hands them to
 // another lane of its wavefront: the step's start y0 and end k5, the stage
 // forces F1 and F6..F13, h and the step's start time, the orbit's output
-// index, the last sample emitted before the step and the lane that owns the
-// orbit.
+// index and the last sample emitted before the step.
 struct Dop853Dense {
     double y0[DIM], k5[DIM];
     double F[9][3];  // F1, F6, F7, ..., F13
     double h, t_step;
     int64_t idx;
-    int32_t ii, owner;
+    int32_t ii;
 };
 
 // stage index of Dop853Dense::F[q]
 DRIFT_HD DRIFT_FI constexpr int dense_force_stage(int q) { return q == 0 ? 1 : q + 5; }
 
+// A Dop853Dense as a record of 64-bit words, the form a wavefront's stash
+// holds: y0, k5, F (row-major), h, t_step, then idx and ii as integers; one
+// word of padding makes the count even (the stash moves words in pairs).
+constexpr int kDenseUsedWords = 2 * DIM + 27 + 4;
+constexpr int kDenseRecordWords = kDenseUsedWords + (kDenseUsedWords & 1);
+
+struct DenseWords {
+    double w[kDenseRecordWords];
+};
+
+DRIFT_HD DRIFT_FI void dense_pack(const Dop853Dense& r, DenseWords& x) {
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        x.w[i] = r.y0[i];
+        x.w[DIM + i] = r.k5[i];
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) x.w[2 * DIM + 3 * q + c] = r.F[q][c];
+    x.w[2 * DIM + 27] = r.h;
+    x.w[2 * DIM + 28] = r.t_step;
+    x.w[2 * DIM + 29] = __builtin_bit_cast(double, (int64_t)r.idx);
+    x.w[2 * DIM + 30] = __builtin_bit_cast(double, (int64_t)r.ii);
+    static_assert(2 * DIM + 30 < kDenseRecordWords, "the record holds every word");
+#pragma unroll
+    for (int k = kDenseUsedWords; k < kDenseRecordWords; k++) x.w[k] = 0.0;
+}
+
+DRIFT_HD DRIFT_FI void dense_unpack(const DenseWords& x, Dop853Dense& r) {
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        r.y0[i] = x.w[i];
+        r.k5[i] = x.w[DIM + i];
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) r.F[q][c] = x.w[2 * DIM + 3 * q + c];
+    r.h = x.w[2 * DIM + 27];
+    r.t_step = x.w[2 * DIM + 28];
+    r.idx = __builtin_bit_cast(int64_t, x.w[2 * DIM + 29]);
+    r.ii = (int32_t)__builtin_bit_cast(int64_t, x.w[2 * DIM + 30]);
+}
+
 // The same attempted step with its dense output handed over instead of
-// written: an accepted step with samples fills a Dop853Dense (owner and idx
-// are left to the caller), the lane's sample index and next sample time move
+// written: an accepted step with samples fills a Dop853Dense (idx is left to
+// the caller), the lane's sample index and next sample time move
 // past the samples the step covers, and round(own, req, completes) runs
 // before the step is committed -- at a point every lane of the caller
 // reaches, with own set where this lane's step left req -- so that whoever
@@ -630,13 +673,6 @@ DRIFT_HD DRIFT_FI bool dop853_attempt_deferred(Dop853Lane& L, const Dop853Job& J
         L.accepted++;
         const double* times = J.times;
         own = L.ii < last && J.pos_neg * L.t_next < J.pos_neg * t_new;
-        // the times of the two samples after the first, fetched before the
-        // FSAL force, which hides their latency
-        double t_pre[2] = {0.0, 0.0};
-        if (own) {
-#pragma unroll
-            for (int q = 0; q < 2; q++) t_pre[q] = L.ii + 2 + q <= last ? times[L.ii + 2 + q] : 0.0;
-        }
         {
             double a[DIM];
             f(t_new, k5, a);
@@ -659,17 +695,13 @@ DRIFT_HD DRIFT_FI bool dop853_attempt_deferred(Dop853Lane& L, const Dop853Job& J
             req.ii = L.ii;
             // the samples the step covers, as dop853_dense_emit's loop walks
             // them: t_s is the time of sample ii + 1 after each
-            int32_t ii = L.ii + 1;
-            double t_s = t_pre[0];
-            if (ii < last && J.pos_neg * t_s < J.pos_neg * t_new) {
+            int32_t ii = L.ii;
+            double t_s;
+            do {
+                const int32_t after = ii + 2;
+                t_s = after <= last ? times[after] : 0.0;
                 ii++;
-                t_s = t_pre[1];
-                while (ii < last && J.pos_neg * t_s < J.pos_neg * t_new) {
-                    const int32_t after = ii + 2;
-                    t_s = after <= last ? times[after] : 0.0;
-                    ii++;
-                }
-            }
+            } while (ii < last && J.pos_neg * t_s < J.pos_neg * t_new);
             L.ii = ii;
             L.t_next = t_s;
         }
@@ -684,7 +716,7 @@ DRIFT_HD DRIFT_FI bool dop853_attempt_deferred(Dop853Lane& L, const Dop853Job& J
         return true;
     }
     if (accept) {
-        dop853_commit_accept(L, J, k5, Fs[13], h, hnew, t_step, t_new);
+        dop853_commit_accept(L, J, k5, Fs[13], h, hnew, t_new);
     } else {
         // t_current and t_old keep their values (dop853.rs:777-778)
         L.reject = 1;

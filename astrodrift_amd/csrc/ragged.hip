@@ -161,7 +161,7 @@ __device__ __forceinline__ void save(const Resident& r, const Dop853Lane& L, int
     }
     r.w(12) = L.h;
     r.w(13) = L.t_cur;
-    r.w(14) = L.t_old;
+    // word 14 is unused (the lane keeps no t_old)
     r.w(15) = pack2(L.ii, L.reject);
     r.w(16) = pack2((int32_t)L.accepted, (int32_t)L.rejected);
     r.w(17) = pack2((int32_t)L.dense, L.status);
@@ -176,7 +176,6 @@ __device__ __forceinline__ void restore(const Resident& r, Dop853Lane& L) {
     }
     L.h = r.w(12);
     L.t_cur = r.w(13);
-    L.t_old = r.w(14);
     int32_t a, b;
     unpack2(r.w(15), L.ii, L.reject);
     unpack2(r.w(16), a, b);

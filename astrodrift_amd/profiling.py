@@ -13,7 +13,8 @@ The run launches, on one stream: a calibration copy of a known size (torch's
 copy of a 2 GiB float64 tensor: 2 GiB read, 2 GiB written), then one
 integration of the 1,048,576-orbit Milky-Way DOP853 workload (rtol = atol =
 1e-12, 1,001 outputs over t = 0..50) with the persistent schedule and the
-default (cost) orbit ordering.  The copy's counter values give the factor that
+default (cost) orbit ordering and block size (512 threads: packed dense
+output; --block 256 runs the inline form).  The copy's counter values give the factor that
 turns FETCH_SIZE / WRITE_SIZE into bytes for the integrator's dispatch.
 
 Summarize (reads the CSV files rocprofv3 wrote, prints markdown):
@@ -29,7 +30,9 @@ import math
 import sys
 from pathlib import Path
 
-INTEGRATOR = "dop853_persistent_kernel"
+# the persistent integrator: dop853_persistent_packed_kernel (the default
+# launch) or, with --block 64..256, dop853_persistent_kernel
+INTEGRATOR = "dop853_persistent"
 COPY = "__amd_rocclr_copyBuffer"
 ALGORITHMIC_BYTES_PER_ORBIT_SAMPLE = 48  # one (x, y, z, vx, vy, vz) record
 
@@ -68,7 +71,8 @@ def run(args) -> int:
     st = torch.zeros((w.n, 4), dtype=torch.int32, device=dev)
     lt = math.log(w.rtol)
     _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(), out.data_ptr(),
-                          w.n, w.nt, lt, lt, w.potential, st.data_ptr(), schedule=args.schedule)
+                          w.n, w.nt, lt, lt, w.potential, st.data_ptr(), schedule=args.schedule,
+                          block_size=args.block)
     torch.cuda.synchronize()
     acc = int(st[:, 0].sum().item())
     print(f"[profiling] integrator: n={w.n} nt={w.nt} accepted={acc} "
@@ -129,6 +133,10 @@ def derived(c: dict[str, float]) -> list[tuple[str, str, str]]:
                      "THREAD_CYCLES_VALU / (64 ACTIVE_INST_VALU)"))
     f64 = [k for k in ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64",
                        "SQ_INSTS_VALU_TRANS_F64") if k in c]
+    if "SQ_WAIT_INST_LDS" in c and "SQ_WAVE_CYCLES" in c:
+        rows.append(("waiting to issue an LDS instruction",
+                     f"{100 * c['SQ_WAIT_INST_LDS'] / c['SQ_WAVE_CYCLES']:.2f}% of wave cycles",
+                     "WAIT_INST_LDS / WAVE_CYCLES"))
     if len(f64) == 4 and "SQ_INSTS_VALU" in c:
         tot = sum(c[k] for k in f64)
         rows.append(("FP64 share of VALU instructions", f"{100 * tot / c['SQ_INSTS_VALU']:.1f}%",
@@ -280,6 +288,8 @@ def main() -> int:
     p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     p.add_argument("--n", type=int, default=1 << 20)
     p.add_argument("--schedule", type=int, default=1, help="0 plain, 1 persistent")
+    p.add_argument("--block", type=int, default=0,
+                   help="threads per block (0: the library default, 512 for this launch)")
     p.add_argument("--copy-gib", type=float, default=2.0, help="0 skips the calibration copy")
     p.add_argument("--summarize", nargs="+", metavar="DIR",
                    help="trace folder, then optionally the fetch and write folders and more "

@@ -41,10 +41,13 @@ def parse_args():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=1 << 20, help="orbits (default 1,048,576)")
-    p.add_argument("--schedules", default="plain,plain:index,persistent:index,persistent,tuned",
-                   help="comma list of schedule[:ordering] to time; schedule plain|persistent|"
-                        "tuned (the autotuner's choice), ordering auto|index|cost (auto: cost "
-                        "order from 16,384 orbits, either schedule)")
+    p.add_argument("--schedules",
+                   default="plain,plain:index,persistent:index,persistent:auto:256,persistent,tuned",
+                   help="comma list of schedule[:ordering[:block]] to time; schedule plain|"
+                        "persistent|tuned (the autotuner's choice), ordering auto|index|cost "
+                        "(auto: cost order from 16,384 orbits, either schedule), block the "
+                        "threads per block (default: the library's, 512 -- packed dense output "
+                        "-- for persistent launches, 256 for plain ones)")
     p.add_argument("--cpu-stride", type=int, default=512,
                    help="CPU baseline integrates every k-th orbit")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -65,13 +68,14 @@ TRAFFIC_PROFILE = REPO / "profiles" / "dop853_mw_1M_traffic.md"
 
 
 def profiled_traffic(n: int, schedule: str, tuned: dict | None):
-    """HBM bytes per launch of the persistent, cost-ordered, 256-thread-block
-    1M-orbit integrator, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (a counter run cannot share a process with the timed loop); None
-    for any other configuration."""
+    """HBM bytes per launch of the default persistent 1M-orbit integrator
+    (cost order, 512-thread blocks with packed dense output, direct output),
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (a counter
+    run cannot share a process with the timed loop); None for any other
+    configuration."""
     same_kernel = schedule == "persistent" or (
-        tuned is not None and tuned["schedule"] == 1 and tuned["block_size"] == 256
-        and tuned["ordering"] == 2)
+        tuned is not None and tuned["schedule"] == 1 and tuned["block_size"] in (0, 512)
+        and tuned["ordering"] == 2 and tuned.get("staging", 1) in (0, 1))
     if n != 1 << 20 or not same_kernel or not TRAFFIC_PROFILE.exists():
         return None
     for line in TRAFFIC_PROFILE.read_text().splitlines():
@@ -505,9 +509,10 @@ def main() -> int:
         # "schedule" or "schedule:ordering"; the default ordering (auto) sorts
         # the orbits by estimated cost; "plain:index" is the reference's
         # launch: one lane per orbit, in input order
-        sched_name, _, order_name = name.partition(":")
+        sched_name, order_name, block_name = (name.split(":") + ["", ""])[:3]
         schedule = sched_ids[sched_name]
         ordering = order_ids[order_name or "auto"]
+        block = int(block_name or 0)
         tuned = None
         if schedule == _native.SCHEDULE_TUNED:
             # untimed: the autotuner times its candidates on the first 5% of
@@ -521,7 +526,7 @@ def main() -> int:
             _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(),
                                   out.data_ptr(), w.n, w.nt, tol_r, tol_a, w.potential,
                                   stats.data_ptr(), schedule=schedule, ordering=ordering,
-                                  stream=stream.cuda_stream)
+                                  block_size=block, stream=stream.cuda_stream)
 
         for _ in range(args.warmup):
             launch()
@@ -540,7 +545,14 @@ def main() -> int:
         st = stats.cpu().numpy()
         accepted = int(st[:, 0].astype(np.int64).sum())
         flop = flops.dop853_flops(st, w.nt, w.potential)
+        if schedule == _native.SCHEDULE_TUNED:
+            shape = (None, tuned["block_size"] or None)
+        else:
+            shape = _native.launch_shape(_native.METHOD_DOP853, w.n, w.potential,
+                                         schedule=schedule, block_size=block)
         results[name] = {
+            "grid": shape[0],
+            "block": shape[1],
             "ms_per_step": wall / args.steps * 1e3,
             "kernel_ms": kernel_s * 1e3,
             "accepted": accepted,

@@ -40,36 +40,41 @@ void run(const F& f, int64_t n, const double* y0, const double* t, int nt, doubl
     }
 }
 
-// The deferred form of the same integration (dop853_attempt_deferred):
-// groups of `lanes` orbits step together; an accepted step with samples
-// leaves a Dop853Dense request, which
-// waits in a stash of `slots` requests until `run_at` are pending (or an orbit
-// of the group completes), as the packed GPU kernels do, and is then run by
-// dop853_run_dense.  The outputs must equal run()'s bit for bit.
+// The deferred form of the same integration (dop853_attempt_deferred), with
+// the packed GPU kernels' policy: groups of `lanes` orbits step together; an
+// accepted step with samples leaves a Dop853Dense request, which waits in a
+// stash of `slots` requests until `run_at` are pending, and then runs
+// (dop853_run_dense) with the group's other requests, the lanes without one
+// of their own taking stashed ones from the top.  An orbit may finish with
+// requests pending: its status is OR-ed into its record, as a non-finite
+// sample's flag is by whoever runs it; a group whose orbits have all finished
+// runs its stash empty.  The outputs must equal run()'s bit for bit.
 template <class F>
 void run_deferred(const F& f, int64_t n, const double* y0, const double* t, int nt, double rtol,
                   double atol, double* out, int64_t* stats, int lanes, int slots, int run_at) {
     const Dop853Job J = make_dop853_job(t, nt, rtol, atol, 1u << 22, t[0], t[nt - 1]);
+    for (int64_t i = 0; i < n; i++) stats[4 * i + 3] = 0;
     for (int64_t g0 = 0; g0 < n; g0 += lanes) {
         const int m = (int)std::min<int64_t>(lanes, n - g0);
         std::vector<Dop853Lane> L(m);
         std::vector<bool> live(m, true);
-        std::vector<int32_t> nf(m, 0);
-        std::vector<Dop853Dense> stash;
+        std::vector<DenseWords> stash;  // requests in the stash's record form
         for (int l = 0; l < m; l++) {
             HostSink sink{out, n, g0 + l};
             dop853_init(L[l], y0 + DIM * (g0 + l), t[0], J, f, sink);
         }
         auto run_req = [&](const Dop853Dense& r) {
             HostSink sink{out, n, r.idx};
-            nf[r.owner] |= dop853_run_dense(r, J, f, sink);
+            stats[4 * r.idx + 3] |= dop853_run_dense(r, J, f, sink);
         };
         for (;;) {
-            std::vector<Dop853Dense> own;
-            std::vector<int> fin;
+            bool any_live = false;
+            for (int l = 0; l < m; l++) any_live = any_live || live[l];
+            if (!any_live && stash.empty()) break;
             // the lanes' attempts; their requests run after all of them (a
             // request writes only its own orbit's samples, so when it runs
             // against the owner's commit does not matter)
+            std::vector<Dop853Dense> own;
             for (int l = 0; l < m; l++) {
                 if (!live[l]) continue;
                 HostSink sink{out, n, g0 + l};
@@ -77,39 +82,33 @@ void run_deferred(const F& f, int64_t n, const double* y0, const double* t, int 
                     L[l], J, f, sink, [&](bool has, Dop853Dense& r, bool) {
                         if (!has) return;
                         r.idx = g0 + l;
-                        r.owner = l;
                         own.push_back(r);
                     });
-                if (done) fin.push_back(l);
+                if (done) {
+                    stats[4 * (g0 + l)] = L[l].accepted;
+                    stats[4 * (g0 + l) + 1] = L[l].rejected;
+                    stats[4 * (g0 + l) + 2] = L[l].dense;
+                    stats[4 * (g0 + l) + 3] |= L[l].status;
+                    live[l] = false;
+                }
             }
             const int k = (int)own.size(), p = (int)stash.size();
-            const bool drain = !fin.empty();
-            if (!drain && k + p < run_at && k + p <= slots) {
-                stash.insert(stash.end(), own.begin(), own.end());
+            if (any_live && k + p < run_at && k + p <= slots) {
+                for (const auto& r : own) {
+                    DenseWords x;
+                    dense_pack(r, x);
+                    stash.push_back(x);
+                }
             } else if (k + p > 0) {
                 for (const auto& r : own) run_req(r);
-                // lanes without a request of their own take stashed ones, from the top
                 int take = std::min(p, lanes - k);
                 while (take-- > 0) {
-                    run_req(stash.back());
+                    Dop853Dense r;
+                    dense_unpack(stash.back(), r);
+                    run_req(r);
                     stash.pop_back();
                 }
-                if (drain)
-                    while (!stash.empty()) {
-                        run_req(stash.back());
-                        stash.pop_back();
-                    }
             }
-            for (int l : fin) {
-                stats[4 * (g0 + l)] = L[l].accepted;
-                stats[4 * (g0 + l) + 1] = L[l].rejected;
-                stats[4 * (g0 + l) + 2] = L[l].dense;
-                stats[4 * (g0 + l) + 3] = L[l].status | nf[l];
-                live[l] = false;
-            }
-            bool any = false;
-            for (int l = 0; l < m; l++) any = any || live[l];
-            if (!any) break;
         }
     }
 }
@@ -193,4 +192,33 @@ extern "C" int host_dop853_deferred(const drift_potential* p, int64_t n, const d
     return with_force(p, [&](const auto& f) {
         run_deferred(f, n, y0, t, nt, rtol, atol, out, stats, lanes, slots, run_at);
     });
+}
+
+// The stash record of a Dop853Dense with the given integers and a pattern of
+// doubles (word k = base + k, so every position is checked), packed and
+// unpacked: returns 0 when every field comes back bit for bit.
+extern "C" int host_dense_roundtrip(int64_t idx, int32_t ii, double base) {
+    Dop853Dense r, back;
+    int k = 0;
+    for (int i = 0; i < DIM; i++) r.y0[i] = base + k++;
+    for (int i = 0; i < DIM; i++) r.k5[i] = base + k++;
+    for (int q = 0; q < 9; q++)
+        for (int c = 0; c < 3; c++) r.F[q][c] = base + k++;
+    r.h = base + k++;
+    r.t_step = base + k++;
+    r.idx = idx;
+    r.ii = ii;
+    DenseWords x;
+    dense_pack(r, x);
+    dense_unpack(x, back);
+    auto same = [](double a, double b) {
+        return __builtin_bit_cast(uint64_t, a) == __builtin_bit_cast(uint64_t, b);
+    };
+    int bad = 0;
+    for (int i = 0; i < DIM; i++) bad += !same(r.y0[i], back.y0[i]) + !same(r.k5[i], back.k5[i]);
+    for (int q = 0; q < 9; q++)
+        for (int c = 0; c < 3; c++) bad += !same(r.F[q][c], back.F[q][c]);
+    bad += !same(r.h, back.h) + !same(r.t_step, back.t_step);
+    bad += (r.idx != back.idx) + (r.ii != back.ii);
+    return bad + (int)sizeof(x.w) / 8 * 0;
 }

@@ -125,6 +125,8 @@ def host_lib(tmp_path_factory):
                                 ctypes.c_void_p]
     lib.host_dop853_deferred.restype = ctypes.c_int
     lib.host_dop853_deferred.argtypes = lib.host_dop853.argtypes + [ctypes.c_int] * 3
+    lib.host_dense_roundtrip.restype = ctypes.c_int
+    lib.host_dense_roundtrip.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_double]
     return lib
 
 
@@ -208,6 +210,16 @@ def test_deferred_dense_output_equals_inline(host_lib, pot, lanes, slots, run_at
         out, st = _host_deferred(host_lib, spec, y0, times, tol, tol, lanes, slots, run_at)
         assert np.array_equal(st, rst)
         assert np.array_equal(out, ref), np.nanmax(np.abs(out - ref))
+
+
+@pytest.mark.parametrize("idx,ii,base", [(0, 0, 0.0), (1 << 40, (1 << 31) - 1, -1e300),
+                                         ((1 << 32) + 7, 1000, float("nan")),
+                                         (123456789, 17, float("inf"))])
+def test_dense_request_record_round_trip(host_lib, idx, ii, base):
+    """A dense-output request survives its stash record (dense_pack /
+    dense_unpack): every double bit for bit, the orbit index up to 2^40 and
+    the sample index up to 2^31 - 1."""
+    assert host_lib.host_dense_roundtrip(idx, ii, base) == 0
 
 
 def test_dopr54_header_is_current():
