@@ -330,7 +330,7 @@ def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
     import numpy as np
     import torch
 
-    from astrodrift_amd import _native, workloads
+    from astrodrift_amd import _native, flops, workloads
 
     dev = torch.device("cuda:0")
     w = workloads.mw_disk(n=n, rtol=1e-7, atol=1e-7)
@@ -379,8 +379,13 @@ def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
             st = stats.cpu().numpy()
             acc = int(st[:, 0].astype(np.int64).sum())
             key = f"{prec_name}_{sched_name}"
+            # algorithmic FLOPs of the reference algorithm (flops.py) over the
+            # launch time; the mixed mode does the force part in FP32, so its
+            # fraction of the FP64 peak is an algorithmic rate, not a unit's
+            tf = flops.dop853_flops(st, w.nt, w.potential) / sec / 1e12
             runs[key] = {"ms": round(sec * 1e3, 3), "accepted": acc,
-                         "value": acc / sec, "bad_status": int(np.count_nonzero(st[:, 3]))}
+                         "value": acc / sec, "bad_status": int(np.count_nonzero(st[:, 3])),
+                         "tflops": round(tf, 3), "frac": round(tf / FP64_PEAK_TFLOPS, 4)}
             if sched_name == "persistent_staged":
                 finals[prec_name] = out[-1].cpu().numpy()
             print(f"[bench] precision {key}: {runs[key]}", file=sys.stderr, flush=True)
@@ -409,6 +414,8 @@ def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
                        "max": float(err.max())}
     best_mixed = max(runs[f"mixed_{k}"]["value"] for k in ("persistent", "persistent_staged", "tuned"))
     best_fp64 = max(runs[f"fp64_{k}"]["value"] for k in ("persistent", "persistent_staged", "tuned"))
+    best_key = max((f"mixed_{k}" for k in ("persistent", "persistent_staged", "tuned")),
+                   key=lambda k: runs[k]["value"])
     return {
         "workload": f"mw_disk_dop853: {w.n} orbits, DOP853 rtol=atol=1e-7, {w.nt} outputs over "
                     "t=0..50; mixed = FP32 stage forces, FP64 state/error norm/step control; "
@@ -423,6 +430,12 @@ def precision_leg(steps: int, warmup: int, n: int, stream, subset: int = 4096):
         "energy_rel_err_t50": energy,
         "position_rel_err_t50_vs_fp64_rtol1e-13": position,
         "position_subset": len(idx),
+        # the fastest mixed run: algorithmic FLOP/s against the FP64 vector
+        # peak; at this tolerance a step emits about three samples, so the
+        # launch is bound by its sample writes as much as by arithmetic
+        "roofline": {"bound": "fp64_valu", "run": best_key,
+                     "achieved": runs[best_key]["tflops"], "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": runs[best_key]["frac"]},
     }
 
 
