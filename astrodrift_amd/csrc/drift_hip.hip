@@ -124,6 +124,22 @@ struct KindForce<KIND, FT, true> {
     }
 };
 
+// The force of a time-windowed moving-origin launch (rows in LDS).
+template <int KIND>
+struct WindowForce {
+    using Static = ForceFor<static_kind(KIND), double>;
+    using type = MovingForce<typename Static::type>;
+    __device__ static type make(const PotParams& p, const double* table, const double* rows,
+                                int32_t win_lo, int32_t win_n) {
+        type f{Static::make(p, table), p.tracks, p.origin_params, p.n_origins, p.n_segments,
+               p.t_start, p.inv_dt};
+        f.win_rows = rows;
+        f.win_lo = win_lo;
+        f.win_n = win_n;
+        return f;
+    }
+};
+
 // Stages the MW coefficient table into LDS (no-op for the other potentials).
 template <int KIND, class FT>
 __device__ __forceinline__ const FT* stage_table(const void* gtable, FT* lds) {
@@ -311,6 +327,148 @@ __global__ void __launch_bounds__(kDefaultBlock) dopr54_persistent_kernel(PlainA
                 st.status = L.err | L.status;
                 a.stats[sink.idx] = st;
             }
+            need = true;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Time-windowed launches for moving origins (DOP853, persistent schedule,
+// two or more origins).  Lanes on unrelated orbits sit at unrelated times, so
+// every origin after the first gathers its 144-byte track row on every force
+// evaluation from 64 different lines per load instruction.  A windowed launch
+// integrates all orbits up to one output sample, then all orbits up to a later
+// one, and so on: one kernel per window, each orbit paused after it emits the
+// window's last sample with its state saved (kResumeWords words, SoA) and
+// resumed by the next window exactly where it stopped, so outputs and
+// statistics are bit-identical to one launch.  Within a window the stage
+// times lie between the window's first and last sample times (plus one step),
+// and the kernel stages those segments' rows of every origin in LDS; a lane
+// whose time falls outside them reads the tracks as before.
+// ---------------------------------------------------------------------------
+constexpr int kResumeWords = 15;
+constexpr int kWindowRowsMax = 96;  // LDS rows a window stages (13.5 KB)
+
+struct WindowArgs {
+    double* state;    // kResumeWords x n, word k of orbit i at k * n + i
+    int32_t first;    // 1: the first window, orbits start from state0
+    int32_t stop;     // pause an orbit once it has emitted sample `stop`
+    int32_t s_begin;  // the previous window's last sample (0 for the first)
+};
+
+__device__ __forceinline__ double pack_words(int32_t a, int32_t b) {
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)a << 32) | (uint32_t)b));
+}
+__device__ __forceinline__ void unpack_words(double w, int32_t& a, int32_t& b) {
+    const uint64_t u = (uint64_t)__double_as_longlong(w);
+    a = (int32_t)(u >> 32);
+    b = (int32_t)(uint32_t)u;
+}
+
+// word 14: 1 while the orbit runs, 2 once it has finished
+__device__ __forceinline__ void resume_save(double* st, int64_t n, int64_t i, const Dop853Lane& L,
+                                            int32_t phase) {
+#pragma unroll
+    for (int k = 0; k < DIM; k++) st[k * n + i] = L.y[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) st[(6 + k) * n + i] = L.F1[k];
+    st[9 * n + i] = L.h;
+    st[10 * n + i] = L.t_cur;
+    st[11 * n + i] = pack_words(L.ii, L.reject);
+    st[12 * n + i] = pack_words((int32_t)L.accepted, (int32_t)L.rejected);
+    st[13 * n + i] = pack_words((int32_t)L.dense, L.status);
+    st[14 * n + i] = pack_words(phase, 0);
+}
+
+__device__ __forceinline__ int32_t resume_load(const double* st, int64_t n, int64_t i,
+                                               Dop853Lane& L) {
+    int32_t phase, pad;
+    unpack_words(st[14 * n + i], phase, pad);
+    if (phase != 1) return phase;
+#pragma unroll
+    for (int k = 0; k < DIM; k++) L.y[k] = st[k * n + i];
+#pragma unroll
+    for (int k = 0; k < 3; k++) L.F1[k] = st[(6 + k) * n + i];
+    L.h = st[9 * n + i];
+    L.t_cur = st[10 * n + i];
+    int32_t a, b;
+    unpack_words(st[11 * n + i], L.ii, L.reject);
+    unpack_words(st[12 * n + i], a, b);
+    L.accepted = (uint32_t)a;
+    L.rejected = (uint32_t)b;
+    unpack_words(st[13 * n + i], a, L.status);
+    L.dense = (uint32_t)a;
+    return phase;
+}
+
+// The track segment of time t, as MovingForce finds it.
+__device__ __forceinline__ int64_t track_segment(const PotParams& p, double t) {
+    double sf = std::floor((t - p.t_start) * p.inv_dt);
+    if (!(sf >= 0.0)) sf = 0.0;
+    const double last = (double)(p.n_segments - 1);
+    return (int64_t)(sf > last ? last : sf);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES
+dop853_window_kernel(PlainArgs a, WindowArgs w) {
+    __shared__ double lds[kind_has_table(KIND) ? TABLE_DOUBLES : 1];
+    __shared__ double rows[kWindowRowsMax * ORIGIN_COEFS];
+    const double* table = stage_table<KIND, double>(a.table, lds);
+    // the segments from the window's first sample time to one past its last
+    // (a step may overshoot the last sample), as many as fit
+    const int64_t seg_lo = track_segment(a.pot, a.times[w.s_begin]);
+    const int64_t seg_hi = track_segment(a.pot, a.times[w.stop]) + 1;
+    const int32_t fit = kWindowRowsMax / max(a.pot.n_origins, 1);
+    const int32_t win_n = (int32_t)max((int64_t)0, min((int64_t)fit, min(seg_hi, (int64_t)a.pot.n_segments - 1) - seg_lo + 1));
+    for (int k = threadIdx.x; k < a.pot.n_origins * win_n * ORIGIN_COEFS; k += blockDim.x) {
+        const int r = k / ORIGIN_COEFS, c = k % ORIGIN_COEFS;
+        const int i = r / win_n, s = r % win_n;
+        rows[k] = a.pot.tracks[((int64_t)i * a.pot.n_segments + seg_lo + s) * ORIGIN_COEFS + c];
+    }
+    __syncthreads();
+    const auto f = WindowForce<KIND>::make(a.pot, table, rows, (int32_t)seg_lo, win_n);
+    const double t0 = a.times[0];
+    const Dop853Job J = make_dop853_job(a.times, a.nt, a.rtol, a.atol, a.max_attempts, t0,
+                                        a.times[a.nt - 1]);
+    OutSink sink{a.out, a.n, 0, false};
+    Dop853Lane L;
+    bool need = true;
+    for (;;) {
+        int64_t orbit = -1;
+        if (!claim_orbit(need, a.queue, a.n, a.order, orbit)) break;
+        if (need) {
+            sink.idx = orbit;
+            need = false;
+            if (w.first) {
+                double y0[DIM];
+                load_state(a.state0, orbit, y0);
+                dop853_init(L, y0, t0, J, f, sink);
+            } else if (resume_load(w.state, a.n, orbit, L) != 1) {
+                need = true;  // finished in an earlier window
+                continue;
+            } else {
+                dop853_refresh_next(L, J);
+            }
+            if (L.ii >= w.stop && L.ii < J.nt - 1) {
+                resume_save(w.state, a.n, orbit, L, 1);
+                need = true;
+                continue;
+            }
+        }
+        if (dop853_attempt(L, J, f, sink)) {
+            if (a.stats) {
+                drift_orbit_stats st;
+                st.accepted = L.accepted;
+                st.rejected = L.rejected;
+                st.dense_steps = L.dense;
+                st.status = L.status;
+                a.stats[sink.idx] = st;
+            }
+            resume_save(w.state, a.n, sink.idx, L, 2);
+            need = true;
+        } else if (L.ii >= w.stop) {
+            resume_save(w.state, a.n, sink.idx, L, 1);
             need = true;
         }
     }
@@ -994,6 +1152,27 @@ int resolve_block(int method, int kind, int block_size, int schedule, bool stage
     return DRIFT_OK;
 }
 
+using WindowKernel = void (*)(PlainArgs, WindowArgs);
+
+WindowKernel window_kernel_for(int kind) {
+    switch (static_kind(kind)) {
+        case POT_KEPLER: return dop853_window_kernel<POT_MOVING | POT_KEPLER>;
+        case POT_PLUMMER: return dop853_window_kernel<POT_MOVING | POT_PLUMMER>;
+        case POT_MW: return dop853_window_kernel<POT_MOVING | POT_MW>;
+        default: return dop853_window_kernel<POT_MOVING | POT_NONE>;
+    }
+}
+
+// Output samples per time window: as many as keep the window's segments
+// (those its samples span, plus one a step may overshoot into, plus one for a
+// window that starts inside a segment) within the rows per origin, taking the
+// time grid to span the tracks; at least one.
+int32_t window_samples(int32_t rows_per_origin, int32_t n_segments, int32_t nt) {
+    const double seg_per_sample = (double)n_segments / (double)std::max(nt - 1, 1);
+    const double k = std::floor((double)(rows_per_origin - 2) / seg_per_sample);
+    return (int32_t)std::max(1.0, std::min(k, (double)std::max(nt - 1, 1)));
+}
+
 using KeyKernel = void (*)(PlainArgs, uint16_t*, uint32_t*, int32_t);
 
 template <int KIND>
@@ -1255,6 +1434,47 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
             DRIFT_HIP_TRY(hipStreamWaitEvent(st, s->staging_done, 0));
         a.out = static_cast<double*>(s->staging.ptr);
         a.staged = 1;
+    }
+    // two or more moving origins, persistent: time windows with the tracks'
+    // rows in LDS (the resumable state is session scratch, so not under
+    // capture)
+    const int32_t rows_per_origin = kWindowRowsMax / std::max(a.pot.n_origins, 1);
+    const bool windowed = method == DRIFT_METHOD_DOP853 && (a.pot.kind & POT_MOVING) &&
+                          a.pot.n_origins >= 2 && rows_per_origin >= 3 &&
+                          opt.schedule == DRIFT_SCHEDULE_PERSISTENT && !capturing && !staged;
+    if (windowed) {
+        if ((rc = ensure_buffer(s->resume, (size_t)kResumeWords * n * sizeof(double))) != DRIFT_OK)
+            return rc;
+        if (!s->resume_done)
+            DRIFT_HIP_TRY(hipEventCreateWithFlags(&s->resume_done, hipEventDisableTiming));
+        else
+            DRIFT_HIP_TRY(hipStreamWaitEvent(st, s->resume_done, 0));
+        WindowKernel wk = window_kernel_for(a.pot.kind);
+        if ((rc = integrator_grid(s->cu_count, (PlainKernel)nullptr, DRIFT_SCHEDULE_PLAIN, block, n,
+                                  blocks)) != DRIFT_OK)
+            return rc;
+        int per_cu = 0;
+        DRIFT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wk, block, 0));
+        blocks = std::min<int64_t>(blocks, (int64_t)std::max(per_cu, 1) * std::max(s->cu_count, 1));
+        const int32_t samples = window_samples(rows_per_origin, a.pot.n_segments, (int32_t)nt);
+        WindowArgs w{static_cast<double*>(s->resume.ptr), 1, 0, 0};
+        for (;;) {
+            w.stop = (int32_t)std::min<int64_t>((int64_t)w.s_begin + samples, (int64_t)nt - 1);
+            if (!w.first) {
+                hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, st, a.queue);
+                DRIFT_HIP_TRY(hipGetLastError());
+            }
+            hipLaunchKernelGGL(wk, dim3((unsigned)blocks), dim3(block), 0, st, a, w);
+            DRIFT_HIP_TRY(hipGetLastError());
+            if (w.stop >= (int32_t)nt - 1) break;
+            w.first = 0;
+            w.s_begin = w.stop;
+        }
+        s->last_grid = (int32_t)blocks;
+        s->last_block = block;
+        if (cost_order) DRIFT_HIP_TRY(hipEventRecord(s->order_done, st));
+        DRIFT_HIP_TRY(hipEventRecord(s->resume_done, st));
+        return DRIFT_OK;
     }
     if (packed && a.stats) {
         hipLaunchKernelGGL(zero_status_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
@@ -1631,12 +1851,13 @@ int drift_session_destroy(drift_session* s) {
     }
     for (DeviceBuffer* b : {&s->state0, &s->times, &s->out, &s->stats, &s->order_keys,
                             &s->order_slots, &s->order_bins, &s->tune_out, &s->tune_stats,
-                            &s->staging, &s->force_io})
+                            &s->staging, &s->force_io, &s->resume})
         if (b->ptr) (void)hipFree(b->ptr);
     for (hipEvent_t e : s->phase_events)
         if (e) (void)hipEventDestroy(e);
     if (s->order_done) (void)hipEventDestroy(s->order_done);
     if (s->staging_done) (void)hipEventDestroy(s->staging_done);
+    if (s->resume_done) (void)hipEventDestroy(s->resume_done);
     if (s->queues) (void)hipFree(s->queues);
     for (auto& r : s->ragged) ragged_scratch_free(r);
     (void)hipStreamDestroy(s->stream);

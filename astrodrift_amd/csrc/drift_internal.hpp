@@ -71,6 +71,10 @@ struct drift_session {
     // staging_done marks the end of the last launch that used it
     drift::DeviceBuffer staging;
     hipEvent_t staging_done = nullptr;
+    // time-windowed moving-origin launches: the orbits' resumable state
+    // between windows; resume_done marks the end of the last launch using it
+    drift::DeviceBuffer resume;
+    hipEvent_t resume_done = nullptr;
     // drift_force's device copies of its host buffers
     drift::DeviceBuffer force_io;
     // moving-origin sets (drift_origins_create)

@@ -143,10 +143,29 @@ struct MovingForce {
     // issued before the static force, whose evaluation hides its latency.
     mutable int64_t row_seg = -1;
     mutable double row[ORIGIN_COEFS];
+    // A window of the tracks staged in LDS by a time-windowed launch: the
+    // rows of segments win_lo .. win_lo + win_n - 1 of every origin, origin
+    // i's segment win_lo + s at win_rows[(i * win_n + s) * ORIGIN_COEFS]
+    // (win_n = 0: no window).  A stage time inside it reads its rows there,
+    // one outside it from the tracks; the arithmetic is the same.
+    const double* win_rows = nullptr;
+    int32_t win_lo = 0, win_n = 0;
 
     DRIFT_HD DRIFT_FI static double quintic(const double* p, double tau) {
         // (tau^5 .. tau^0) coefficients
         return fma(fma(fma(fma(fma(p[0], tau, p[1]), tau, p[2]), tau, p[3]), tau, p[4]), tau, p[5]);
+    }
+
+    DRIFT_HD DRIFT_FI void add_origin(const double* c, int32_t i, const double* q, double tau,
+                                      double& ax, double& ay, double& az) const {
+        const double* pb =
+            static_cast<const double*>(__builtin_assume_aligned(origin_params + 2 * i, 16));
+        double fx, fy, fz;
+        plummer_acc<double>(pb[0], pb[1], q[0] - quintic(c, tau), q[1] - quintic(c + 6, tau),
+                            q[2] - quintic(c + 12, tau), fx, fy, fz);
+        ax += fx;
+        ay += fy;
+        az += fz;
     }
 
     DRIFT_HD DRIFT_FI void operator()(double t, const double* q, double* a) const {
@@ -157,12 +176,22 @@ struct MovingForce {
         if (sf > last) sf = last;
         const double tau = u - sf;
         const int64_t seg = (int64_t)sf;
+        const int64_t ws = seg - win_lo;
+        const bool in_win = (uint64_t)ws < (uint64_t)win_n;
         if (n_origins > 0 && seg != row_seg) {
             // rows are 144 bytes from a 256-byte aligned base: 16-byte loads
-            const double* c = static_cast<const double*>(
-                __builtin_assume_aligned(tracks + seg * ORIGIN_COEFS, 16));
+            // (LDS and global memory on separate paths, no generic loads)
+            if (in_win) {
+                const double* c = static_cast<const double*>(
+                    __builtin_assume_aligned(win_rows + ws * ORIGIN_COEFS, 16));
 #pragma unroll
-            for (int k = 0; k < ORIGIN_COEFS; k++) row[k] = c[k];
+                for (int k = 0; k < ORIGIN_COEFS; k++) row[k] = c[k];
+            } else {
+                const double* c = static_cast<const double*>(
+                    __builtin_assume_aligned(tracks + seg * ORIGIN_COEFS, 16));
+#pragma unroll
+                for (int k = 0; k < ORIGIN_COEFS; k++) row[k] = c[k];
+            }
             row_seg = seg;
         }
         base(t, q, a);
@@ -172,17 +201,16 @@ struct MovingForce {
             plummer_acc<double>(pb[0], pb[1], q[0] - quintic(row, tau), q[1] - quintic(row + 6, tau),
                                 q[2] - quintic(row + 12, tau), ax, ay, az);
         }
-        for (int32_t i = 1; i < n_origins; ++i) {
-            const double* c = static_cast<const double*>(__builtin_assume_aligned(
-                tracks + ((int64_t)i * n_segments + seg) * ORIGIN_COEFS, 16));
-            const double* pb =
-                static_cast<const double*>(__builtin_assume_aligned(origin_params + 2 * i, 16));
-            double fx, fy, fz;
-            plummer_acc<double>(pb[0], pb[1], q[0] - quintic(c, tau), q[1] - quintic(c + 6, tau),
-                                q[2] - quintic(c + 12, tau), fx, fy, fz);
-            ax += fx;
-            ay += fy;
-            az += fz;
+        if (in_win) {
+            for (int32_t i = 1; i < n_origins; ++i)
+                add_origin(static_cast<const double*>(__builtin_assume_aligned(
+                               win_rows + ((int64_t)i * win_n + ws) * ORIGIN_COEFS, 16)),
+                           i, q, tau, ax, ay, az);
+        } else {
+            for (int32_t i = 1; i < n_origins; ++i)
+                add_origin(static_cast<const double*>(__builtin_assume_aligned(
+                               tracks + ((int64_t)i * n_segments + seg) * ORIGIN_COEFS, 16)),
+                           i, q, tau, ax, ay, az);
         }
         a[3] += ax;
         a[4] += ay;

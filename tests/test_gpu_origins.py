@@ -76,6 +76,57 @@ def test_moving_origin_parity(gpu, method, static):
     assert abs(acc_g - acc_c) <= 1e-3 * acc_c
 
 
+def satellites(k, t1=10.0, nseg=100, amp=0.02, b=0.05) -> Origins:
+    """k Plummer spheres on inclined circles of radii 0.8..1.2 at spread
+    phases (k origins on the same segments)."""
+    times = np.linspace(0.0, t1, nseg + 1)
+    r = np.linspace(0.8, 1.2, k)
+    om = 2 * math.pi / 9.0 / r ** 1.5
+    th = om[None, :] * times[:, None] + np.linspace(0.0, 2 * math.pi, k, endpoint=False)[None, :]
+    x = np.stack([r * np.cos(th), r * np.sin(th), 0.1 * r * np.sin(th)], axis=-1)
+    v = np.stack([-r * om * np.sin(th), r * om * np.cos(th), 0.1 * r * om * np.cos(th)], axis=-1)
+    a = -(om ** 2)[None, :, None] * x
+    return Origins.hermite(np.concatenate([x, v], axis=-1), a, times, (amp, b))
+
+
+@pytest.mark.parametrize("static", ["mw", "none"])
+@pytest.mark.parametrize("k,nseg,nt", [(3, 100, 401), (16, 200, 101)])
+def test_windowed_launch_equals_one_launch(gpu, static, k, nseg, nt):
+    """Two or more origins: a persistent DOP853 launch runs in time windows
+    with the tracks' rows in LDS, pausing every orbit after each window's last
+    sample and resuming it in the next (3 origins: windows of about 190
+    samples; 16 origins: 50 windows of 2 samples, where many steps overshoot
+    the staged rows and read the tracks).  Bit for bit the plain schedule's
+    single launch, step counts included, and the oracle on a subset."""
+    import torch
+
+    spec = STATICS[static].with_origins(satellites(k, nseg=nseg))
+    y0 = orbits_near_track(8192, seed=5)
+    times = np.linspace(0.0, 10.0, nt)
+    tol = math.log(1e-10)
+    d_y0 = torch.from_numpy(y0).to(gpu)
+    d_t = torch.from_numpy(times).to(gpu)
+    res = {}
+    for schedule in (_native.SCHEDULE_PLAIN, _native.SCHEDULE_PERSISTENT):
+        d_out = torch.full((nt, y0.shape[0], 6), float("nan"), dtype=torch.float64, device=gpu)
+        d_st = torch.zeros((y0.shape[0], 4), dtype=torch.int32, device=gpu)
+        print(f"windowed moving origins: k={k} static={static} schedule={schedule}", flush=True)
+        _native.launch_device(_native.METHOD_DOP853, d_y0.data_ptr(), d_t.data_ptr(),
+                              d_out.data_ptr(), y0.shape[0], nt, tol, tol, spec, d_st.data_ptr(),
+                              schedule=schedule, ordering=_native.ORDER_COST)
+        torch.cuda.synchronize()
+        res[schedule] = (d_out.cpu().numpy(), d_st.cpu().numpy())
+    (po, ps), (qo, qs) = res[_native.SCHEDULE_PLAIN], res[_native.SCHEDULE_PERSISTENT]
+    assert not np.isnan(po).any() and (ps[:, 3] == 0).all()
+    assert np.array_equal(qo.view(np.uint64), po.view(np.uint64))
+    assert np.array_equal(qs, ps)
+    idx = np.arange(0, y0.shape[0], 256)
+    cpu, _ = oracle_lib.batch(_native.METHOD_DOP853, y0[idx], times, tol, tol, spec, nthreads=8)
+    ep, ev = rel_err(qo[:, idx], cpu)
+    print(f"  pos_rel={ep:.3e} vel_rel={ev:.3e}")
+    assert ep < REL_TOL and ev < REL_TOL
+
+
 @pytest.mark.parametrize("schedule", [_native.SCHEDULE_PLAIN, _native.SCHEDULE_PERSISTENT])
 def test_moving_origin_device_schedules(gpu, schedule):
     import torch
