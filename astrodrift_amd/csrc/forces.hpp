@@ -15,6 +15,8 @@ namespace drift {
 
 template <class FT>
 struct GalpyKeplerForce {
+    // a point mass: few registers (the DOPR54 step estimate runs out of line)
+    static constexpr bool kFewRegisters = true;
     DRIFT_HD DRIFT_FI void operator()(double, const double* q, double* a) const {
         FT ax, ay, az;
         galpy_kepler_acc<FT>(FT(q[0]), FT(q[1]), FT(q[2]), ax, ay, az);
@@ -29,6 +31,8 @@ struct GalpyKeplerForce {
 
 template <class FT>
 struct KeplerForce {
+    // a point mass: few registers (the DOPR54 step estimate runs out of line)
+    static constexpr bool kFewRegisters = true;
     FT amp;
     DRIFT_HD DRIFT_FI void operator()(double, const double* q, double* a) const {
         FT ax, ay, az;
@@ -44,6 +48,8 @@ struct KeplerForce {
 
 template <class FT>
 struct PlummerForce {
+    // a point mass: few registers (the DOPR54 step estimate runs out of line)
+    static constexpr bool kFewRegisters = true;
     FT amp, b2;
     DRIFT_HD DRIFT_FI void operator()(double, const double* q, double* a) const {
         FT ax, ay, az;
@@ -110,6 +116,8 @@ struct MwForce {
 // No static force: the background of a potential made of moving origins only
 // (or free motion).
 struct NoForce {
+    // a point mass: few registers (the DOPR54 step estimate runs out of line)
+    static constexpr bool kFewRegisters = true;
     DRIFT_HD DRIFT_FI void operator()(double, const double* q, double* a) const {
         a[0] = q[3];
         a[1] = q[4];

@@ -34,6 +34,8 @@
 // the samples not reached.
 #pragma once
 
+#include <type_traits>
+
 #include "drift_device.hpp"
 #include "dop853_rkn.hpp"
 #include "dopr54_rkn.hpp"
@@ -812,8 +814,9 @@ DRIFT_HD DRIFT_FI double max_abs6(const double* y) {
 // rk4_estimate_step, dopr54.rs:210-368, with the scale e^s = atol + rtol max|y|.
 // Each step-doubling test takes three rk4_onestep calls (dopr54.rs:135-208):
 // dt from yo, dt/2 from yo, dt/2 from that half step.  Their 12 stages run as
-// one loop, so the force is inlined once: the estimate runs once per orbit
-// but its register demand sets the whole kernel's.  The arithmetic and its
+// one loop, so the force is inlined once (for point masses the estimate is
+// called out of line, rk4_estimate_call, so that its registers stay out of
+// the step loop's budget).  The arithmetic and its
 // order are rk4_onestep's: stage k of a step of length h from `base` at tn is
 // taken at tn, tn + h/2, tn + h/2, tn + h, adds h a / (6, 3, 3, 6) to the
 // step's accumulator, and the next stage point is base + (h a) (1/2, 1/2, 1).
@@ -872,6 +875,33 @@ DRIFT_HD DRIFT_FI double rk4_estimate_step(const double* yo, double dt, double t
     return dt;
 }
 
+// rk4_estimate_step as a call of its own, the state passed by value: the
+// estimate runs once per orbit and needs some 45 more registers than the
+// step loop, so inlined it would set the kernel's register count and with
+// it the occupancy (the kernels' waves-per-SIMD target then bounds the call's
+// registers as well, and it spills inside itself if it must).  Only the
+// job's tolerances are read.  Used for the forces that declare
+// kFewRegisters (point masses), whose launches gain a wave per SIMD by it;
+// the others keep the estimate inline (a table or an origin track cache
+// passed by value would cost more than the call saves).
+template <class F, class = void>
+struct few_registers : std::false_type {};
+template <class F>
+struct few_registers<F, std::void_t<decltype(F::kFewRegisters)>>
+    : std::bool_constant<F::kFewRegisters> {};
+
+template <class F>
+DRIFT_HD __attribute__((noinline)) double rk4_estimate_call(double y0, double y1, double y2,
+                                                            double y3, double y4, double y5,
+                                                            double dt, double to, double atol,
+                                                            double rtol, F f) {
+    const double yo[DIM] = {y0, y1, y2, y3, y4, y5};
+    Dopr54Job J{};
+    J.atol = atol;
+    J.rtol = rtol;
+    return rk4_estimate_step(yo, dt, to, J, f);
+}
+
 // Writes samples for every interval the lane has completed (the save after
 // each dopr54_onestep, dopr54.rs:728-764) and opens the next interval.
 template <class Sink>
@@ -896,7 +926,13 @@ DRIFT_HD DRIFT_FI void dopr54_init(Dopr54Lane& L, const double* y0, double t0, c
     for (int i = 0; i < DIM; i++) L.y[i] = y0[i];
     sink.emit(0, L.y);
     double dt_one = J.dt_one_init;
-    if (dt_one == dp54::initial_step_sentinel) dt_one = rk4_estimate_step(L.y, J.dt, t0, J, f);
+    if (dt_one == dp54::initial_step_sentinel) {
+        if constexpr (few_registers<F>::value)
+            dt_one = rk4_estimate_call(L.y[0], L.y[1], L.y[2], L.y[3], L.y[4], L.y[5], J.dt, t0,
+                                       J.atol, J.rtol, f);
+        else
+            dt_one = rk4_estimate_step(L.y, J.dt, t0, J, f);
+    }
     L.to = t0;
     f(t0, L.y, L.a1);
     L.dt_one = dt_one;
