@@ -602,7 +602,8 @@ def launch_shape(method: int, n: int, potential: PotentialSpec | None = None,
                  precision: int = PRECISION_FP64, schedule: int | None = None,
                  block_size: int = 0, device: int = 0) -> tuple[int, int]:
     """drift_launch_shape: (grid, block) of the integrator kernel for n orbits;
-    schedule None is the host-buffer entry points' (persistent) shape."""
+    schedule None is the host-buffer entry points' shape (persistent for
+    DOP853, plain for DOPR54)."""
     s = session(device)
     if potential is None:
         pot = None

@@ -166,7 +166,8 @@ typedef enum drift_schedule {
     DRIFT_SCHEDULE_PERSISTENT = 1, /* persistent kernel, lanes refilled from a work queue */
     DRIFT_SCHEDULE_TUNED = 2       /* the options drift_tune_device stored in the session for
                                       this (method, potential kind, precision, log2 n); without
-                                      one, PERSISTENT with AUTO ordering */
+                                      one, the method's default (DOP853 PERSISTENT, DOPR54
+                                      PLAIN) with AUTO ordering */
 } drift_schedule;
 
 typedef struct drift_session drift_session;
@@ -258,7 +259,8 @@ int drift_galpy_dop853_device_noalloc(const double* d_state0, const double* d_ti
 
 /* The integrator kernel's grid (blocks) and block (threads) for a launch of
  * n orbits: what drift_galpy_*_device would launch with these options
- * (options NULL: the host-buffer entry points' persistent schedule).  The
+ * (options NULL: the host-buffer entry points' schedule, persistent for DOP853
+ * and plain for DOPR54).  The
  * persistent schedule's grid is the device's resident block count, capped by
  * n.  Allocates nothing; the reference's fixed shape is ceil(n / 128) x 128
  * (src/dispatch/gpu.rs:23-29). */

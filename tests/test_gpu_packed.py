@@ -115,6 +115,17 @@ def test_packed_is_the_persistent_default(gpu):
     assert block == 256
 
 
+def test_default_schedules_of_the_host_buffer_entry_points(gpu):
+    """No options: DOP853 launches persistent (a grid of resident blocks),
+    DOPR54 plain (one lane per orbit, ceil(n / 256) blocks)."""
+    n = 1 << 20
+    w = workloads.mw_disk(n=256, t_end=1.0, steps=11)
+    grid, block = _native.launch_shape(_native.METHOD_DOPR54, n, w.potential)
+    assert (grid, block) == ((n + 255) // 256, 256)
+    grid, block = _native.launch_shape(_native.METHOD_DOP853, n, w.potential)
+    assert block == 512 and grid < n // 512
+
+
 def test_packed_is_refused_where_it_does_not_exist(gpu):
     """block_size 512 needs DOP853 and a static potential."""
     w = workloads.mw_disk(n=256, t_end=1.0, steps=11)
