@@ -64,6 +64,8 @@ def run(args) -> int:
 
     if args.ragged_jobs > 0:
         return run_ragged(args.ragged_jobs)
+    if args.dopr54_uniform > 0:
+        return run_dopr54_uniform(args.dopr54_uniform)
     w = workloads.mw_disk(n=args.n)
     s0 = torch.from_numpy(w.state0).to(dev)
     tt = torch.from_numpy(w.times).to(dev)
@@ -115,6 +117,35 @@ def run_ragged(njobs: int) -> int:
                            else flops.dopr54_flops(s, spec))
     print(f"[profiling] ragged mix: {njobs} jobs, {sum(j.n for j in jobs)} orbits, algorithmic "
           f"FLOPs DOP853 {flop['DOP853']:.4e} DOPR54 {flop['DOPR54']:.4e}", flush=True)
+    return 0
+
+
+def run_dopr54_uniform(n: int) -> int:
+    """DOPR54 on n disk orbits under each potential of the ragged mix (Kepler,
+    Plummer, Milky Way; rtol = atol = 1e-10, 101 outputs over t = 0..50), one
+    launch each with the default (plain, cost-ordered) schedule.  Prints each
+    launch's accepted steps and algorithmic FLOPs."""
+    import torch
+
+    from . import _native, flops, workloads
+
+    dev = torch.device("cuda:0")
+    times = workloads.grid_times(0.0, 50.0, 101)
+    tt = torch.from_numpy(times).to(dev)
+    lt = math.log(1e-10)
+    for k, spec in enumerate(workloads.RAGGED_POTENTIALS):
+        y0 = workloads.disk_orbits(spec, n, seed=7 + k)
+        s0 = torch.from_numpy(y0).to(dev)
+        out = torch.empty((len(times), n, 6), dtype=torch.float64, device=dev)
+        st = torch.zeros((n, 4), dtype=torch.int32, device=dev)
+        _native.launch_device(_native.METHOD_DOPR54, s0.data_ptr(), tt.data_ptr(), out.data_ptr(),
+                              n, len(times), lt, lt, spec, st.data_ptr())
+        torch.cuda.synchronize()
+        s = st.cpu().numpy()
+        print(f"[profiling] DOPR54 {('Kepler', 'Plummer', 'Milky Way')[k]}: n={n} "
+              f"accepted={int(s[:, 0].astype('int64').sum())} "
+              f"algorithmic FLOPs={flops.dopr54_flops(s, spec):.4e}", flush=True)
+        del out, st, s0
     return 0
 
 
@@ -297,6 +328,9 @@ def main() -> int:
     p.add_argument("--ragged-jobs", type=int, default=0,
                    help="run the first N jobs of the ragged mix (one drift_ragged_device call) "
                         "instead of the 1M-orbit launch")
+    p.add_argument("--dopr54-uniform", type=int, default=0, metavar="N",
+                   help="run DOPR54 on N disk orbits under each ragged-mix potential instead "
+                        "of the 1M-orbit launch")
     p.add_argument("--kernel", metavar="NAME",
                    help="with --summarize: the SQ counters of the dispatches whose name contains "
                         "NAME, from every folder given, with derived ratios")

@@ -211,6 +211,21 @@ class RaggedJob:
         return self.times.shape[0]
 
 
+def disk_orbits(spec: PotentialSpec, n: int, seed: int) -> np.ndarray:
+    """n disk orbits of the ragged mix's distribution in one potential: R
+    uniform in [0.3, 2], z ~ N(0, 0.03), circular speed with 5% scatter,
+    radial and vertical speeds ~ N(0, 0.05); (n, 6) AoS."""
+    rng = np.random.default_rng(seed)
+    R = rng.uniform(0.3, 2.0, n)
+    phi = rng.uniform(0.0, 2.0 * math.pi, n)
+    z = rng.normal(0.0, 0.03, n)
+    vphi = vcirc(spec, R) * (1.0 + rng.normal(0.0, 0.05, n))
+    vR, vz = rng.normal(0.0, 0.05, n), rng.normal(0.0, 0.05, n)
+    c, s = np.cos(phi), np.sin(phi)
+    return np.ascontiguousarray(
+        np.stack([R * c, R * s, z, vR * c - vphi * s, vR * s + vphi * c, vz], axis=1))
+
+
 def ragged_mix(njobs: int = 4096, seed: int = RAGGED_SEED) -> list[RaggedJob]:
     rng = np.random.default_rng(seed)
     jobs = []
