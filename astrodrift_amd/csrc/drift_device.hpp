@@ -53,13 +53,18 @@ constexpr int ORIGIN_COEFS = 18;
 // t in [0, 1) inside the segment.  Each segment holds TABLE_NCOEF monomial
 // coefficients of a Chebyshev interpolant in the local variable
 // s = 2t - 1 in [-1, 1].  The LDS table covers r2 in [2^TABLE_E_MIN,
-// 2^TABLE_E_MAX) (r from 2.4e-4 to 4,096; 60 KB).  A second, "far" table of the
-// same layout covers every positive finite double r2, [2^-1074, 2^1024)
-// (subnormal r2 are scaled by 2^64 to find their key); it stays in global
-// memory and is read only when an orbit leaves the LDS range.
+// 2^TABLE_E_MAX) (r from 2^-6 = 0.016 to 64 model units, 0.12 to 512 kpc at
+// R0 = 8 kpc; 30 KB).  A second, "far" table of the same layout covers every
+// positive finite double r2, [2^-1074, 2^1024) (subnormal r2 are scaled by
+// 2^64 to find their key); it stays in global memory and is read only when an
+// orbit leaves the LDS range.  The two are built segment by segment by the
+// same routine, so on the octaves they share their coefficients, and the
+// force, are the same bits: the LDS range only decides which lanes read
+// global memory.  Half of the former 48 octaves kept the same results and
+// left room for larger dense-output stashes and a third wave per SIMD.
 constexpr int TABLE_SUBBITS = 4;
-constexpr int TABLE_E_MIN = -24;
-constexpr int TABLE_E_MAX = 24;
+constexpr int TABLE_E_MIN = -12;
+constexpr int TABLE_E_MAX = 12;
 constexpr int TABLE_DEG = 9;
 constexpr int TABLE_NCOEF = TABLE_DEG + 1;
 constexpr int TABLE_SEGMENTS = (TABLE_E_MAX - TABLE_E_MIN) << TABLE_SUBBITS;

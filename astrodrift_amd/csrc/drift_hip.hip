@@ -167,8 +167,18 @@ constexpr int kDefaultBlock = 256;
 // two costs a small register spill and measured ~10-30% faster on gfx950.
 #define DRIFT_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
 
-template <int KIND, class FT>
-__global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_plain_kernel(PlainArgs a) {
+// The plain DOP853 kernel also exists at three waves per SIMD (WAVES = 3) for
+// the FP64 Milky-Way force: 168 VGPRs with 9 spilled, three 256-thread
+// blocks' 30 KB tables per CU.  It is launched from kWidePlainMinOrbits
+// orbits on, where the grid fills the GPU's 768 block slots at least twice:
+// the plain launch then ran 3-4.5% faster than at two waves (524,288 to
+// 1,048,576 orbits); below, the blocks' last round left slots empty and two
+// waves were faster (65,536 orbits: 3%, 262,144: 7%).
+constexpr size_t kWidePlainMinOrbits = 393216;
+
+template <int KIND, class FT, int WAVES = 2>
+__global__ void __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
+dop853_plain_kernel(PlainArgs a) {
     __shared__ FT lds[kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1];
     const FT* table = stage_table<KIND, FT>(a.table, lds);
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -198,8 +208,9 @@ __global__ void __launch_bounds__(kDefaultBlock) DRIFT_TWO_WAVES dop853_plain_ke
 // Waves per SIMD of the DOPR54 kernels.  Without a table the step loop needs
 // under 104 registers (the step estimate runs out of line), so four waves fit
 // and hide the latency of the short force evaluations (a point-mass launch
-// runs 14-24% faster than at three); the float table's 12 KB of LDS allow
-// three, the FP64 table's 60 KB two, as do the moving origins' registers.
+// runs 14-24% faster than at three); with the Milky-Way table (30 KB in FP64,
+// 6 KB in float) the kernels need 154-160 VGPRs and run three; the moving
+// origins' registers allow two.
 template <int KIND, class FT>
 constexpr int dopr54_waves() {
     return (KIND & POT_MOVING) ? 2 : kind_has_table(KIND) ? (sizeof(FT) == 8 ? 2 : 3) : 4;
@@ -1102,7 +1113,10 @@ namespace {
 using PlainKernel = void (*)(PlainArgs);
 
 template <int KIND, class FT>
-PlainKernel pick_schedule(int method, int schedule) {
+PlainKernel pick_schedule(int method, int schedule, size_t n) {
+    if constexpr (KIND == POT_MW && sizeof(FT) == 8)
+        if (method == DRIFT_METHOD_DOP853 && !schedule && n >= kWidePlainMinOrbits)
+            return dop853_plain_kernel<KIND, FT, 3>;
     if (method == DRIFT_METHOD_DOP853)
         return schedule ? dop853_persistent_kernel<KIND, FT> : dop853_plain_kernel<KIND, FT>;
     return schedule ? dopr54_persistent_kernel<KIND, FT> : dopr54_plain_kernel<KIND, FT>;
@@ -1116,29 +1130,30 @@ PlainKernel pick_packed(int schedule) {
 }
 
 template <int KIND>
-PlainKernel pick_plain(int method, int precision, int schedule, bool packed) {
+PlainKernel pick_plain(int method, int precision, int schedule, bool packed, size_t n) {
     if (packed)
         return precision == DRIFT_PRECISION_MIXED ? pick_packed<KIND, float>(schedule)
                                                   : pick_packed<KIND, double>(schedule);
-    return precision == DRIFT_PRECISION_MIXED ? pick_schedule<KIND, float>(method, schedule)
-                                              : pick_schedule<KIND, double>(method, schedule);
+    return precision == DRIFT_PRECISION_MIXED ? pick_schedule<KIND, float>(method, schedule, n)
+                                              : pick_schedule<KIND, double>(method, schedule, n);
 }
 
 // Moving-origin kinds exist in FP64 only (launch_integrator refuses MIXED),
 // and without packed dense output.
-PlainKernel kernel_for(int method, int kind, int precision, int schedule, bool packed = false) {
+PlainKernel kernel_for(int method, int kind, int precision, int schedule, bool packed, size_t n) {
     switch (kind) {
         case POT_GALPY_KEPLER:
-            return pick_plain<POT_GALPY_KEPLER>(method, precision, schedule, packed);
-        case POT_KEPLER: return pick_plain<POT_KEPLER>(method, precision, schedule, packed);
-        case POT_PLUMMER: return pick_plain<POT_PLUMMER>(method, precision, schedule, packed);
-        case POT_MW: return pick_plain<POT_MW>(method, precision, schedule, packed);
+            return pick_plain<POT_GALPY_KEPLER>(method, precision, schedule, packed, n);
+        case POT_KEPLER: return pick_plain<POT_KEPLER>(method, precision, schedule, packed, n);
+        case POT_PLUMMER: return pick_plain<POT_PLUMMER>(method, precision, schedule, packed, n);
+        case POT_MW: return pick_plain<POT_MW>(method, precision, schedule, packed, n);
         case POT_MOVING | POT_KEPLER:
-            return pick_schedule<POT_MOVING | POT_KEPLER, double>(method, schedule);
+            return pick_schedule<POT_MOVING | POT_KEPLER, double>(method, schedule, n);
         case POT_MOVING | POT_PLUMMER:
-            return pick_schedule<POT_MOVING | POT_PLUMMER, double>(method, schedule);
-        case POT_MOVING | POT_MW: return pick_schedule<POT_MOVING | POT_MW, double>(method, schedule);
-        default: return pick_schedule<POT_MOVING | POT_NONE, double>(method, schedule);
+            return pick_schedule<POT_MOVING | POT_PLUMMER, double>(method, schedule, n);
+        case POT_MOVING | POT_MW:
+            return pick_schedule<POT_MOVING | POT_MW, double>(method, schedule, n);
+        default: return pick_schedule<POT_MOVING | POT_NONE, double>(method, schedule, n);
     }
 }
 
@@ -1413,7 +1428,7 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     if ((rc = resolve_block(method, a.pot.kind, opt.block_size, opt.schedule,
                             opt.staging == DRIFT_STAGING_STAGED, block, packed)) != DRIFT_OK)
         return rc;
-    PlainKernel k = kernel_for(method, a.pot.kind, opt.precision, opt.schedule, packed);
+    PlainKernel k = kernel_for(method, a.pot.kind, opt.precision, opt.schedule, packed, n);
     int64_t blocks = 0;
     if ((rc = integrator_grid(s->cu_count, k, opt.schedule, block, n, blocks)) != DRIFT_OK) return rc;
     if (opt.schedule == 1) {
@@ -1793,7 +1808,8 @@ int noalloc_dop853(const double* d_state0, const double* d_times, double* d_out,
     if ((rc = resolve_block(DRIFT_METHOD_DOP853, a.pot.kind, opt.block_size, opt.schedule, false,
                             block, packed)) != DRIFT_OK)
         return rc;
-    PlainKernel k = kernel_for(DRIFT_METHOD_DOP853, a.pot.kind, opt.precision, opt.schedule, packed);
+    PlainKernel k =
+        kernel_for(DRIFT_METHOD_DOP853, a.pot.kind, opt.precision, opt.schedule, packed, n);
     int64_t blocks = 0;
     if ((rc = integrator_grid(d->cu_count, k, opt.schedule, block, n, blocks)) != DRIFT_OK)
         return rc;
@@ -1970,7 +1986,7 @@ int drift_launch_shape(drift_session* s, int32_t method, const drift_potential* 
         return rc;
     DRIFT_HIP_TRY(hipSetDevice(s->device));
     int64_t blocks = 0;
-    rc = integrator_grid(s->cu_count, kernel_for(method, kind, opt.precision, opt.schedule, packed),
+    rc = integrator_grid(s->cu_count, kernel_for(method, kind, opt.precision, opt.schedule, packed, n),
                          opt.schedule, b, n, blocks);
     if (rc != DRIFT_OK) return rc;
     if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");
