@@ -2307,6 +2307,14 @@ int drift_mw_table_check(const drift_potential* potential, int32_t samples_per_s
     std::vector<double> coeffs(TABLE_DOUBLES), far(FAR_DOUBLES);
     build_mw_table(*potential, coeffs.data());
     build_mw_far_table(*potential, far.data());
+    // a lane outside the LDS table's range reads the far table: on the
+    // octaves the two share, the coefficients must be the same bits, so that
+    // the range only decides where a lane reads, never what it computes
+    const size_t shift = (size_t)(TABLE_E_MIN - FAR_E_MIN) << TABLE_SUBBITS;
+    if (std::memcmp(coeffs.data(), far.data() + shift * TABLE_NCOEF,
+                    sizeof(double) * TABLE_DOUBLES) != 0)
+        return fail(DRIFT_ERR_INVALID_ARGUMENT,
+                    "the LDS and far Milky-Way tables differ on a segment they share");
     *max_rel_error = std::max(
         mw_table_max_rel_error(*potential, coeffs.data(), samples_per_segment),
         mw_far_table_max_rel_error(*potential, far.data(), samples_per_segment));

@@ -468,7 +468,10 @@ int drift_force(drift_session* session, const drift_potential* potential, const 
  * error of the tabulated S(r^2) against its long-double evaluation over
  * samples_per_segment points per table segment, over both the LDS table
  * (r^2 in [2^-12, 2^12)) and the far table (every positive finite r^2, where S
- * is a normal double).  No device needed. */
+ * is a normal double).  Also checks that the two tables hold the same
+ * coefficients, bit for bit, on the segments they share (a kernel lane reads
+ * the far table outside the LDS range); DRIFT_ERR_INVALID_ARGUMENT if not.
+ * No device needed. */
 int drift_mw_table_check(const drift_potential* potential, int32_t samples_per_segment,
                          double* max_rel_error);
 

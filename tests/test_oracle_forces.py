@@ -160,7 +160,10 @@ def test_lower_gamma_against_scipy(a):
 
 
 def test_mw_table_check_on_host():
-    """The kernels' tabulated bulge+halo force against its long-double form."""
+    """The kernels' tabulated bulge+halo force against its long-double form;
+    the check also fails unless the LDS table's coefficients equal the far
+    table's, bit for bit, on the octaves they share (the LDS range then only
+    decides which lanes read global memory)."""
     assert _native.mw_table_check(MW, 400) < 1e-15
     # a different parameter set builds its own table
     other = _native.PotentialSpec.milky_way(bulge_alpha=1.2, bulge_rc=0.5, halo_a=0.8)
