@@ -169,12 +169,15 @@ constexpr int kDefaultBlock = 256;
 
 // The plain DOP853 kernel also exists at three waves per SIMD (WAVES = 3) for
 // the FP64 Milky-Way force: 168 VGPRs with 9 spilled, three 256-thread
-// blocks' 30 KB tables per CU.  It is launched from kWidePlainMinOrbits
-// orbits on, where the grid fills the GPU's 768 block slots at least twice:
-// the plain launch then ran 3-4.5% faster than at two waves (524,288 to
-// 1,048,576 orbits); below, the blocks' last round left slots empty and two
-// waves were faster (65,536 orbits: 3%, 262,144: 7%).
-constexpr size_t kWidePlainMinOrbits = 393216;
+// blocks' 30 KB tables per CU.  It is launched when the grid fills the
+// device's block slots at three waves (3 per CU) at least twice
+// (wide_plain): on 256 CUs, from 393,216 orbits, where the plain launch ran
+// 3-4.5% faster than at two waves (524,288 to 1,048,576 orbits); below, the
+// blocks' last round left slots empty and two waves were faster (65,536
+// orbits: 3%, 262,144: 7%).
+static bool wide_plain(size_t n, int cu_count) {
+    return n >= (size_t)2 * 3 * (size_t)std::max(cu_count, 1) * kDefaultBlock;
+}
 
 template <int KIND, class FT, int WAVES = 2>
 __global__ void __launch_bounds__(kDefaultBlock) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
@@ -1113,9 +1116,9 @@ namespace {
 using PlainKernel = void (*)(PlainArgs);
 
 template <int KIND, class FT>
-PlainKernel pick_schedule(int method, int schedule, size_t n) {
+PlainKernel pick_schedule(int method, int schedule, bool wide) {
     if constexpr (KIND == POT_MW && sizeof(FT) == 8)
-        if (method == DRIFT_METHOD_DOP853 && !schedule && n >= kWidePlainMinOrbits)
+        if (method == DRIFT_METHOD_DOP853 && !schedule && wide)
             return dop853_plain_kernel<KIND, FT, 3>;
     if (method == DRIFT_METHOD_DOP853)
         return schedule ? dop853_persistent_kernel<KIND, FT> : dop853_plain_kernel<KIND, FT>;
@@ -1130,30 +1133,30 @@ PlainKernel pick_packed(int schedule) {
 }
 
 template <int KIND>
-PlainKernel pick_plain(int method, int precision, int schedule, bool packed, size_t n) {
+PlainKernel pick_plain(int method, int precision, int schedule, bool packed, bool wide) {
     if (packed)
         return precision == DRIFT_PRECISION_MIXED ? pick_packed<KIND, float>(schedule)
                                                   : pick_packed<KIND, double>(schedule);
-    return precision == DRIFT_PRECISION_MIXED ? pick_schedule<KIND, float>(method, schedule, n)
-                                              : pick_schedule<KIND, double>(method, schedule, n);
+    return precision == DRIFT_PRECISION_MIXED ? pick_schedule<KIND, float>(method, schedule, wide)
+                                              : pick_schedule<KIND, double>(method, schedule, wide);
 }
 
 // Moving-origin kinds exist in FP64 only (launch_integrator refuses MIXED),
 // and without packed dense output.
-PlainKernel kernel_for(int method, int kind, int precision, int schedule, bool packed, size_t n) {
+PlainKernel kernel_for(int method, int kind, int precision, int schedule, bool packed, bool wide) {
     switch (kind) {
         case POT_GALPY_KEPLER:
-            return pick_plain<POT_GALPY_KEPLER>(method, precision, schedule, packed, n);
-        case POT_KEPLER: return pick_plain<POT_KEPLER>(method, precision, schedule, packed, n);
-        case POT_PLUMMER: return pick_plain<POT_PLUMMER>(method, precision, schedule, packed, n);
-        case POT_MW: return pick_plain<POT_MW>(method, precision, schedule, packed, n);
+            return pick_plain<POT_GALPY_KEPLER>(method, precision, schedule, packed, wide);
+        case POT_KEPLER: return pick_plain<POT_KEPLER>(method, precision, schedule, packed, wide);
+        case POT_PLUMMER: return pick_plain<POT_PLUMMER>(method, precision, schedule, packed, wide);
+        case POT_MW: return pick_plain<POT_MW>(method, precision, schedule, packed, wide);
         case POT_MOVING | POT_KEPLER:
-            return pick_schedule<POT_MOVING | POT_KEPLER, double>(method, schedule, n);
+            return pick_schedule<POT_MOVING | POT_KEPLER, double>(method, schedule, wide);
         case POT_MOVING | POT_PLUMMER:
-            return pick_schedule<POT_MOVING | POT_PLUMMER, double>(method, schedule, n);
+            return pick_schedule<POT_MOVING | POT_PLUMMER, double>(method, schedule, wide);
         case POT_MOVING | POT_MW:
-            return pick_schedule<POT_MOVING | POT_MW, double>(method, schedule, n);
-        default: return pick_schedule<POT_MOVING | POT_NONE, double>(method, schedule, n);
+            return pick_schedule<POT_MOVING | POT_MW, double>(method, schedule, wide);
+        default: return pick_schedule<POT_MOVING | POT_NONE, double>(method, schedule, wide);
     }
 }
 
@@ -1428,7 +1431,8 @@ int launch_integrator(drift_session* s, int method, const double* d_state0, cons
     if ((rc = resolve_block(method, a.pot.kind, opt.block_size, opt.schedule,
                             opt.staging == DRIFT_STAGING_STAGED, block, packed)) != DRIFT_OK)
         return rc;
-    PlainKernel k = kernel_for(method, a.pot.kind, opt.precision, opt.schedule, packed, n);
+    PlainKernel k = kernel_for(method, a.pot.kind, opt.precision, opt.schedule, packed,
+                               wide_plain(n, s->cu_count));
     int64_t blocks = 0;
     if ((rc = integrator_grid(s->cu_count, k, opt.schedule, block, n, blocks)) != DRIFT_OK) return rc;
     if (opt.schedule == 1) {
@@ -1809,7 +1813,8 @@ int noalloc_dop853(const double* d_state0, const double* d_times, double* d_out,
                             block, packed)) != DRIFT_OK)
         return rc;
     PlainKernel k =
-        kernel_for(DRIFT_METHOD_DOP853, a.pot.kind, opt.precision, opt.schedule, packed, n);
+        kernel_for(DRIFT_METHOD_DOP853, a.pot.kind, opt.precision, opt.schedule, packed,
+                   wide_plain(n, d->cu_count));
     int64_t blocks = 0;
     if ((rc = integrator_grid(d->cu_count, k, opt.schedule, block, n, blocks)) != DRIFT_OK)
         return rc;
@@ -1986,7 +1991,9 @@ int drift_launch_shape(drift_session* s, int32_t method, const drift_potential* 
         return rc;
     DRIFT_HIP_TRY(hipSetDevice(s->device));
     int64_t blocks = 0;
-    rc = integrator_grid(s->cu_count, kernel_for(method, kind, opt.precision, opt.schedule, packed, n),
+    rc = integrator_grid(s->cu_count,
+                         kernel_for(method, kind, opt.precision, opt.schedule, packed,
+                                    wide_plain(n, s->cu_count)),
                          opt.schedule, b, n, blocks);
     if (rc != DRIFT_OK) return rc;
     if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "n too large");

@@ -61,6 +61,21 @@ def test_mw_dop853_every_order_is_bit_identical(gpu):
         assert np.array_equal(st, ref_st), (schedule, ordering)
 
 
+def test_three_wave_plain_kernel_is_bit_identical(gpu):
+    """A plain FP64 Milky-Way launch large enough to fill the block slots twice
+    at three waves per SIMD (400,000 orbits on 256 CUs) runs the three-wave
+    plain kernel; its trajectories and statistics equal the persistent
+    schedule's, bit for bit, in index and in cost order."""
+    w = workloads.mw_disk(n=400000, t_end=1.0, steps=21)
+    ref_out, ref_st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_PERSISTENT,
+                           _native.ORDER_INDEX)
+    assert not np.isnan(ref_out).any()
+    for ordering in (_native.ORDER_INDEX, _native.ORDER_COST):
+        out, st = _run(w, _native.METHOD_DOP853, _native.SCHEDULE_PLAIN, ordering)
+        assert _same(out, ref_out), ordering
+        assert np.array_equal(st, ref_st), ordering
+
+
 def test_kepler_dopr54_cost_order_is_bit_identical(gpu):
     w = workloads.kepler_disk(n=20000)
     ref_out, ref_st = _run(w, _native.METHOD_DOPR54, _native.SCHEDULE_PLAIN, _native.ORDER_INDEX)
