@@ -491,6 +491,17 @@ def session(device: int = 0) -> Session:
         return s
 
 
+def release_session(device: int = 0) -> None:
+    """Destroys the process-wide session for `device`, freeing the device
+    memory it caches (launch outputs of the host-buffer entry points, the
+    staged-output buffer, tuner scratch, force tables); the next call creates
+    a new one.  Close BoundPotential objects of the session first."""
+    with _sessions_lock:
+        s = _sessions.pop(device, None)
+    if s is not None:
+        s.close()
+
+
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
     return ctypes.c_void_p(a.ctypes.data)
 

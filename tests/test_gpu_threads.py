@@ -84,3 +84,17 @@ def test_threads_sharing_a_session_match_sequential_launches(gpu):
         assert np.array_equal(out.view(np.uint64), e_out.view(np.uint64)), (tid, rep, k)
         assert np.array_equal(stats, e_stats), (tid, rep, k)
         assert np.array_equal(acc, force_expected), (tid, rep)
+
+
+def test_released_session_is_replaced_by_a_new_one(gpu):
+    """release_session frees the process-wide session; the next call builds a
+    new one whose launches give the same bits."""
+    w = workloads.mw_disk(n=2000, t_end=2.0, steps=21)
+    lt = math.log(w.rtol)
+    first = _native.launch_host(_native.METHOD_DOP853, w.state0, w.times, lt, lt, w.potential)
+    old = _native.session(0)
+    _native.release_session(0)
+    assert not old.handle
+    again = _native.launch_host(_native.METHOD_DOP853, w.state0, w.times, lt, lt, w.potential)
+    assert _native.session(0) is not old
+    assert np.array_equal(np.asarray(first[0]).view(np.uint64), np.asarray(again[0]).view(np.uint64))
