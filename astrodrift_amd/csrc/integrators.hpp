@@ -919,20 +919,15 @@ DRIFT_HD DRIFT_FI void dopr54_emit_completed(Dopr54Lane& L, const Dopr54Job& J, 
     }
 }
 
+// The start of an orbit once its initial sub-step dt_one is decided (the
+// job's explicit value or the RK4 estimate): sample 0, the first force, the
+// bookkeeping, and the samples an already completed interval owes.
 template <class F, class Sink>
-DRIFT_HD DRIFT_FI void dopr54_init(Dopr54Lane& L, const double* y0, double t0, const Dopr54Job& J,
-                                   const F& f, Sink& sink) {
+DRIFT_HD DRIFT_FI void dopr54_begin(Dopr54Lane& L, const double* y0, double t0, double dt_one,
+                                    const Dopr54Job& J, const F& f, Sink& sink) {
 #pragma unroll
     for (int i = 0; i < DIM; i++) L.y[i] = y0[i];
     sink.emit(0, L.y);
-    double dt_one = J.dt_one_init;
-    if (dt_one == dp54::initial_step_sentinel) {
-        if constexpr (few_registers<F>::value)
-            dt_one = rk4_estimate_call(L.y[0], L.y[1], L.y[2], L.y[3], L.y[4], L.y[5], J.dt, t0,
-                                       J.atol, J.rtol, f);
-        else
-            dt_one = rk4_estimate_step(L.y, J.dt, t0, J, f);
-    }
     L.to = t0;
     f(t0, L.y, L.a1);
     L.dt_one = dt_one;
@@ -943,6 +938,21 @@ DRIFT_HD DRIFT_FI void dopr54_init(Dopr54Lane& L, const double* y0, double t0, c
     L.accepted = L.rejected = 0;
     L.status = 0;
     dopr54_emit_completed(L, J, sink);
+}
+
+// dopr54_begin after the estimate the sentinel asks for (dopr54.rs:713-726).
+template <class F, class Sink>
+DRIFT_HD DRIFT_FI void dopr54_init(Dopr54Lane& L, const double* y0, double t0, const Dopr54Job& J,
+                                   const F& f, Sink& sink) {
+    double dt_one = J.dt_one_init;
+    if (dt_one == dp54::initial_step_sentinel) {
+        if constexpr (few_registers<F>::value)
+            dt_one = rk4_estimate_call(y0[0], y0[1], y0[2], y0[3], y0[4], y0[5], J.dt, t0, J.atol,
+                                       J.rtol, f);
+        else
+            dt_one = rk4_estimate_step(y0, J.dt, t0, J, f);
+    }
+    dopr54_begin(L, y0, t0, dt_one, J, f, sink);
 }
 
 // One pass of the dopr54_onestep loop body (dopr54.rs:403-459) with

@@ -50,6 +50,10 @@ struct RaggedArgs {
     double bound2[6];
     int32_t lds_pot;       // potential whose MW table is staged in LDS, -1 = none
     const void* lds_src;   // that table in global memory (FP64 or the mixed mode's FP32)
+    // DOPR54: each work item's initial sub-step, written by the estimate
+    // kernel before the integrator runs (items whose job passes an explicit
+    // dt_one_init, and orbits that resume, are not written)
+    double* dt_est;
 };
 
 // Per-method launch scratch: the device copy of the descriptors (jobs | pots |
@@ -58,6 +62,7 @@ struct RaggedArgs {
 // counter.
 struct RaggedScratch {
     DeviceBuffer desc;
+    DeviceBuffer dt_est;  // DOPR54: RaggedArgs::dt_est
     char* pinned = nullptr;
     size_t pinned_bytes = 0;
     hipEvent_t upload_done = nullptr;
@@ -71,6 +76,7 @@ struct RaggedScratch {
 struct RaggedLaunch {
     RaggedArgs args{};
     void (*kernel)(RaggedArgs) = nullptr;
+    void (*estimate)(RaggedArgs) = nullptr;  // DOPR54: fills args.dt_est first
     int64_t blocks = 0;
     int block = 0;
     size_t bytes = 0;  // descriptor bytes to upload

@@ -265,7 +265,8 @@ int run_chunk(drift_server* S, std::vector<std::shared_ptr<Request>>& active, hi
                            (uint64_t)a.njobs, (uint64_t)a.max_attempts, (uint64_t)a.items,
                            (uint64_t)(uintptr_t)a.pots, (uint64_t)(uintptr_t)a.tables64,
                            (uint64_t)(uintptr_t)a.tables32, (uint64_t)(uintptr_t)a.queue,
-                           (uint64_t)(int64_t)a.lds_pot, (uint64_t)(uintptr_t)a.lds_src})
+                           (uint64_t)(int64_t)a.lds_pot, (uint64_t)(uintptr_t)a.lds_src,
+                           (uint64_t)(uintptr_t)a.dt_est, (uint64_t)(uintptr_t)L.estimate})
             key.push_back(v);
     }
     key.push_back((uint64_t)(uintptr_t)S->done_pinned);
