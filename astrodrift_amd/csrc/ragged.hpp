@@ -63,6 +63,9 @@ struct RaggedArgs {
 struct RaggedScratch {
     DeviceBuffer desc;
     DeviceBuffer dt_est;  // DOPR54: RaggedArgs::dt_est
+    // the autotuner's timing of the integrator kernel alone (eager launches)
+    bool time_kernel = false;
+    hipEvent_t kernel_start = nullptr, kernel_end = nullptr;
     char* pinned = nullptr;
     size_t pinned_bytes = 0;
     hipEvent_t upload_done = nullptr;
