@@ -82,6 +82,20 @@ constexpr int FAR_SEGMENTS = (FAR_E_MAX - FAR_E_MIN) << TABLE_SUBBITS;
 constexpr int FAR_DOUBLES = FAR_SEGMENTS * TABLE_NCOEF;
 static_assert((TABLE_NCOEF * 8) % 16 == 0, "segments must stay 16-byte aligned for 128-bit loads");
 
+// Cost ordering's buckets: 1/16 octave of an orbit's step rate, the highest
+// rate in bucket 0.  A zero, negative or non-finite rate (no force, a singular
+// or non-finite start: such orbits end at once or are free drifts) takes the
+// last, cheapest bucket.
+constexpr int kOrderBins = 1024;
+constexpr int kOrderBinsPerOctave = 16;
+
+DRIFT_HD DRIFT_FI uint32_t order_bin_of_rate(double rate) {
+    if (!(rate > 0.0) || !(rate < INFINITY)) return kOrderBins - 1;
+    const int level = (int)std::floor(kOrderBinsPerOctave * std::log2(rate));
+    const int bin = kOrderBins / 2 - 1 - level;
+    return (uint32_t)(bin < 0 ? 0 : (bin > kOrderBins - 1 ? kOrderBins - 1 : bin));
+}
+
 // Potential parameters in the form the kernels use (derived on the host
 // from drift_potential).
 struct PotParams {

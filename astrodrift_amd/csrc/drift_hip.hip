@@ -792,8 +792,6 @@ __global__ void __launch_bounds__(256) unstage_kernel(const double* __restrict__
 // idle in the dense-output branch) and measured 1.5-2.7% faster at rtol 1e-7.
 // Buckets are 1/16 octave of the rate.
 // ---------------------------------------------------------------------------
-constexpr int kOrderBins = 1024;
-constexpr int kOrderBinsPerOctave = 16;
 constexpr double kCrossingWeight = 0.25;
 
 // Vertical energy E_z = v_z^2 / 2 + Phi(R, z) - Phi(R, 0) of a Milky-Way
@@ -824,12 +822,7 @@ __device__ __forceinline__ uint32_t cost_bin(const PotParams& p, const double* y
             if (ez > 0.0) rate += kCrossingWeight * std::sqrt(2.0 * ez / p.disk_b2);
         }
     }
-    // no force, a singular or non-finite start: cheapest bin (such orbits end
-    // at once or are free drifts)
-    if (!(rate > 0.0) || !(rate < INFINITY)) return kOrderBins - 1;
-    const int level = (int)std::floor(kOrderBinsPerOctave * std::log2(rate));
-    const int bin = kOrderBins / 2 - 1 - level;
-    return (uint32_t)std::min(std::max(bin, 0), kOrderBins - 1);
+    return order_bin_of_rate(rate);
 }
 
 __global__ void order_reset_kernel(uint32_t* bins) {

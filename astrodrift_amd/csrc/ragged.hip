@@ -18,6 +18,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "drift_internal.hpp"
 #include "forces.hpp"
 #include "integrators.hpp"
@@ -270,7 +272,8 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dop853_kernel(R
         if (!claim_item(need, a.queue, (unsigned long long)a.items, slot)) break;
         if (need) {
             const RaggedJob& d = a.jobs[find_job(a.jobs, a.njobs, slot)];
-            const int64_t orbit = (int64_t)(slot - (unsigned long long)d.item_begin);
+            const int64_t orbit = a.order ? (int64_t)a.order[slot]
+                                          : (int64_t)(slot - (unsigned long long)d.item_begin);
             res = Resident{d.resident, d.n, orbit};
             const int64_t phase = d.resident ? __double_as_longlong(res.w(kResidentPhaseWord))
                                              : kOrbitNew;
@@ -336,7 +339,8 @@ __global__ void __launch_bounds__(256) DRIFT_RAGGED_WAVES ragged_dopr54_kernel(R
         bool done = false;
         if (need) {
             const RaggedJob& d = a.jobs[find_job(a.jobs, a.njobs, slot)];
-            const int64_t orbit = (int64_t)(slot - (unsigned long long)d.item_begin);
+            const int64_t orbit = a.order ? (int64_t)a.order[slot]
+                                          : (int64_t)(slot - (unsigned long long)d.item_begin);
             res = Resident{d.resident, d.n, orbit};
             const int64_t phase = d.resident ? __double_as_longlong(res.w(kResidentPhaseWord))
                                              : kOrbitNew;
@@ -405,7 +409,7 @@ __global__ void __launch_bounds__(256) ragged_dopr54_estimate_kernel(RaggedArgs 
     if (slot >= a.items) return;
     const RaggedJob& d = a.jobs[find_job(a.jobs, a.njobs, (unsigned long long)slot)];
     if (d.dt_one_init != dp54::initial_step_sentinel) return;
-    const int64_t orbit = slot - d.item_begin;
+    const int64_t orbit = a.order ? (int64_t)a.order[slot] : slot - d.item_begin;
     if (d.resident) {
         const Resident res{d.resident, d.n, orbit};
         if (__double_as_longlong(res.w(kResidentPhaseWord)) != kOrbitNew) return;
@@ -424,6 +428,42 @@ __global__ void __launch_bounds__(256) ragged_dopr54_estimate_kernel(RaggedArgs 
     J.atol = d.atol;
     a.dt_est[slot] = rk4_estimate_step(y0, J.dt, d.times[0], J, f);
 }
+
+// Sort keys of a launch's work items for cost order: (job index, bucket of
+// the orbit's step rate at t0, omega = sqrt(|a| / r), highest first) for the
+// jobs of at least min_orbits orbits, (job index, 0) for the others, which so
+// keep index order (the sort is stable); orbits[] holds each item's orbit.
+// Sorted, the items stay grouped by job in job order.  A job that alone fills
+// the device's lanes otherwise runs in index order, and lanes that draw its
+// long orbits late keep the launch running after the others have finished.
+template <class FT>
+__global__ void __launch_bounds__(256) ragged_order_key_kernel(RaggedArgs a, uint32_t* keys,
+                                                               uint32_t* orbits,
+                                                               int64_t min_orbits) {
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= a.items) return;
+    const int32_t jix = find_job(a.jobs, a.njobs, (unsigned long long)slot);
+    const RaggedJob& d = a.jobs[jix];
+    const int64_t orbit = slot - d.item_begin;
+    uint32_t bin = 0;
+    if (d.n >= min_orbits) {
+        LaneForce<FT> f{};
+        set_lane_force<FT>(f, a, d.pot);
+        f.in_lds = false;
+        f.wave_kind = -1;
+        double y[DIM], acc[DIM];
+        const double2* p = reinterpret_cast<const double2*>(d.state0 + orbit * DIM);
+        const double2 u = p[0], v = p[1], w = p[2];
+        y[0] = u.x; y[1] = u.y; y[2] = v.x; y[3] = v.y; y[4] = w.x; y[5] = w.y;
+        f(d.times[0], y, acc);
+        const double r2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+        const double a2 = acc[3] * acc[3] + acc[4] * acc[4] + acc[5] * acc[5];
+        bin = order_bin_of_rate(std::sqrt(std::sqrt(a2 / r2)));
+    }
+    keys[slot] = ((uint32_t)jix << 10) | bin;
+    orbits[slot] = (uint32_t)orbit;
+}
+static_assert(kOrderBins == 1 << 10, "a sort key holds the bucket in its low 10 bits");
 
 __global__ void ragged_reset_queue(unsigned long long* queue) { *queue = 0ull; }
 
@@ -446,7 +486,7 @@ __global__ void ragged_clear_u32(unsigned int* p, int64_t n) {
 int ragged_prepare(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
                    const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
                    const std::vector<const void*>& t32, const drift_launch_options& opt,
-                   RaggedScratch& scratch, RaggedLaunch& launch) {
+                   RaggedScratch& scratch, RaggedLaunch& launch, int64_t order_min_orbits) {
     launch = RaggedLaunch{};
     if (njobs == 0) return DRIFT_OK;
     const int64_t items = host_jobs[njobs - 1].item_begin + host_jobs[njobs - 1].n;
@@ -511,6 +551,36 @@ int ragged_prepare(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, 
         if (rc != DRIFT_OK) return rc;
         a.dt_est = static_cast<double*>(scratch.dt_est.ptr);
     }
+    // cost order when a job alone fills the device's lanes (and the job index
+    // and the bucket fit a 32-bit key)
+    int job_bits = 0;
+    while (job_bits < 31 && (int64_t(1) << job_bits) < (int64_t)njobs) job_bits++;
+    bool order = order_min_orbits > 0 && job_bits + 10 <= 32;
+    if (order) {
+        order = false;
+        for (int32_t k = 0; k < njobs; ++k) order = order || host_jobs[k].n >= order_min_orbits;
+    }
+    if (order) {
+        launch.key_kernel = mixed ? ragged_order_key_kernel<float> : ragged_order_key_kernel<double>;
+        launch.order_min_orbits = order_min_orbits;
+        launch.sort_end_bit = job_bits + 10;
+        size_t temp = 0;
+        DRIFT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+            nullptr, temp, (const uint32_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)nullptr,
+            (uint32_t*)nullptr, items, 0, launch.sort_end_bit));
+        const size_t words = (size_t)items * sizeof(uint32_t);
+        const size_t arrays = (4 * words + 255) & ~size_t(255);
+        rc = ensure_buffer(scratch.sort, arrays + temp);
+        if (rc != DRIFT_OK) return rc;
+        char* b = static_cast<char*>(scratch.sort.ptr);
+        launch.keys_in = reinterpret_cast<uint32_t*>(b);
+        launch.keys_out = reinterpret_cast<uint32_t*>(b + words);
+        launch.orbits_in = reinterpret_cast<uint32_t*>(b + 2 * words);
+        launch.orbits_out = reinterpret_cast<uint32_t*>(b + 3 * words);
+        launch.sort_temp = b + arrays;
+        launch.sort_temp_bytes = temp;
+        a.order = launch.orbits_out;
+    }
     launch.block = opt.block_size ? opt.block_size : 256;
     int per_cu = 0;
     DRIFT_HIP_TRY(
@@ -530,6 +600,17 @@ int ragged_enqueue(const RaggedLaunch& launch, RaggedScratch& scratch, hipStream
     if (!capturing) DRIFT_HIP_TRY(hipEventRecord(scratch.upload_done, stream));
     hipLaunchKernelGGL(ragged_reset_queue, dim3(1), dim3(1), 0, stream, launch.args.queue);
     DRIFT_HIP_TRY(hipGetLastError());
+    if (launch.key_kernel) {
+        const int64_t blocks = (launch.args.items + 255) / 256;
+        if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "too many work items");
+        hipLaunchKernelGGL(launch.key_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                           launch.args, launch.keys_in, launch.orbits_in, launch.order_min_orbits);
+        DRIFT_HIP_TRY(hipGetLastError());
+        size_t temp = launch.sort_temp_bytes;
+        DRIFT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+            launch.sort_temp, temp, launch.keys_in, launch.keys_out, launch.orbits_in,
+            launch.orbits_out, launch.args.items, 0, launch.sort_end_bit, stream));
+    }
     if (launch.estimate) {
         const int64_t blocks = (launch.args.items + 255) / 256;
         if (blocks > INT32_MAX) return fail(DRIFT_ERR_INVALID_ARGUMENT, "too many work items");
@@ -553,9 +634,10 @@ int ragged_enqueue(const RaggedLaunch& launch, RaggedScratch& scratch, hipStream
 int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
                   const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
                   const std::vector<const void*>& t32, const drift_launch_options& opt,
-                  hipStream_t stream, RaggedScratch& scratch) {
+                  hipStream_t stream, RaggedScratch& scratch, int64_t order_min_orbits) {
     RaggedLaunch launch;
-    int rc = ragged_prepare(s, host_jobs, njobs, method, pots, t64, t32, opt, scratch, launch);
+    int rc = ragged_prepare(s, host_jobs, njobs, method, pots, t64, t32, opt, scratch, launch,
+                            order_min_orbits);
     if (rc != DRIFT_OK) return rc;
     return ragged_enqueue(launch, scratch, stream, false);
 }
@@ -567,6 +649,9 @@ void ragged_scratch_free(RaggedScratch& scratch) {
     if (scratch.dt_est.ptr) (void)hipFree(scratch.dt_est.ptr);
     scratch.dt_est.ptr = nullptr;
     scratch.dt_est.bytes = 0;
+    if (scratch.sort.ptr) (void)hipFree(scratch.sort.ptr);
+    scratch.sort.ptr = nullptr;
+    scratch.sort.bytes = 0;
     if (scratch.pinned) (void)hipHostFree(scratch.pinned);
     scratch.pinned = nullptr;
     scratch.pinned_bytes = 0;
@@ -724,8 +809,11 @@ extern "C" int drift_ragged_device(drift_session* session, const drift_job* jobs
         }
         drift_launch_options mo = opt;
         if (!mo.block_size) mo.block_size = session->ragged_block[opt.precision][method];
+        // cost order inside the jobs that alone fill the device's lanes (2
+        // waves of 64 per SIMD, 4 SIMDs per CU)
+        const int64_t lanes = (int64_t)std::max(session->cu_count, 1) * 4 * 2 * 64;
         rc = ragged_launch(session, rj.data(), (int32_t)rj.size(), method, pots, t64, t32, mo, st,
-                           session->ragged[method]);
+                           session->ragged[method], lanes);
         if (rc != DRIFT_OK) {
             // a launch of the other method may already use the outputs and the
             // session's descriptors: let it finish before reporting the error

@@ -54,6 +54,9 @@ struct RaggedArgs {
     // kernel before the integrator runs (items whose job passes an explicit
     // dt_one_init, and orbits that resume, are not written)
     double* dt_est;
+    // work item -> orbit within the item's job (cost order inside the jobs
+    // that alone fill the device's lanes), or NULL: item - job.item_begin
+    const uint32_t* order;
 };
 
 // Per-method launch scratch: the device copy of the descriptors (jobs | pots |
@@ -63,6 +66,7 @@ struct RaggedArgs {
 struct RaggedScratch {
     DeviceBuffer desc;
     DeviceBuffer dt_est;  // DOPR54: RaggedArgs::dt_est
+    DeviceBuffer sort;    // cost order: keys and orbits (twice each) and the sort's scratch
     // the autotuner's timing of the integrator kernel alone (eager launches)
     bool time_kernel = false;
     hipEvent_t kernel_start = nullptr, kernel_end = nullptr;
@@ -80,21 +84,31 @@ struct RaggedLaunch {
     RaggedArgs args{};
     void (*kernel)(RaggedArgs) = nullptr;
     void (*estimate)(RaggedArgs) = nullptr;  // DOPR54: fills args.dt_est first
+    // cost order (drift_ragged_device only): keys from key_kernel for the
+    // jobs of at least order_min_orbits orbits, sorted with their orbits
+    void (*key_kernel)(RaggedArgs, uint32_t*, uint32_t*, int64_t) = nullptr;
+    int64_t order_min_orbits = 0;
+    uint32_t *keys_in = nullptr, *keys_out = nullptr, *orbits_in = nullptr, *orbits_out = nullptr;
+    void* sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+    int sort_end_bit = 0;
     int64_t blocks = 0;
     int block = 0;
     size_t bytes = 0;  // descriptor bytes to upload
 };
 
+// order_min_orbits > 0: cost-order the work items of every job of at least
+// that many orbits (eager launches only: the sort is not captured).
 int ragged_prepare(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
                    const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
                    const std::vector<const void*>& t32, const drift_launch_options& opt,
-                   RaggedScratch& scratch, RaggedLaunch& launch);
+                   RaggedScratch& scratch, RaggedLaunch& launch, int64_t order_min_orbits = 0);
 int ragged_enqueue(const RaggedLaunch& launch, RaggedScratch& scratch, hipStream_t stream,
                    bool capturing);
 int ragged_launch(drift_session* s, const RaggedJob* host_jobs, int32_t njobs, int method,
                   const std::vector<PotParams>& pots, const std::vector<const void*>& t64,
                   const std::vector<const void*>& t32, const drift_launch_options& opt,
-                  hipStream_t stream, RaggedScratch& scratch);
+                  hipStream_t stream, RaggedScratch& scratch, int64_t order_min_orbits = 0);
 void ragged_scratch_free(RaggedScratch& scratch);
 int ragged_clear(double* resident, int64_t n, unsigned int* done_count, hipStream_t stream);
 int resolve_pots(drift_session* s, const drift_potential* potentials, size_t npots,

@@ -316,3 +316,25 @@ def test_ragged_mix_draw_matches_oracle(gpu):
     assert worst < 1e-8
     assert chaotic <= 0.03 * checked
     assert flips <= 0.03 * checked
+
+
+def test_job_filling_the_device_is_cost_ordered_without_changing_bits(gpu):
+    """A job of 160,000 orbits fills the device's lanes (2 waves of 64 per
+    SIMD on 256 CUs: 131,072), so its work items run in cost order; the same
+    orbits as two jobs of 80,000, which stay in index order, give the same
+    trajectories and statistics bit for bit, for both methods."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(91)
+    n = 160000
+    y0 = disk_ics(POTS[2], n, rng)
+    times = workloads.grid_times(0.0, 2.0, 11)
+    for method in (_native.METHOD_DOPR54, _native.METHOD_DOP853):
+        whole = dict(n=n, pot=2, method=method, times=times, tol=1e-9, y0=y0)
+        halves = [dict(n=n // 2, pot=2, method=method, times=times, tol=1e-9,
+                       y0=np.ascontiguousarray(y0[k * (n // 2):(k + 1) * (n // 2)]))
+                  for k in range(2)]
+        (out, st), = run_ragged([whole], torch)
+        parts = run_ragged(halves, torch)
+        assert not np.isnan(out).any()
+        np.testing.assert_array_equal(out, np.concatenate([p[0] for p in parts], axis=1))
+        np.testing.assert_array_equal(st, np.concatenate([p[1] for p in parts], axis=0))
