@@ -321,8 +321,11 @@ typedef struct drift_job {
 
 /* Integrates every job to completion: one persistent launch per method, lanes
  * refilled from a work queue across all jobs of that method (jobs grouped by
- * potential kind, costliest first).  options->schedule is ignored.  Returns
- * after the work is complete (it synchronises `stream`). */
+ * potential kind, costliest first; the orbits of a job that alone fills the
+ * device's lanes in cost order, those of smaller jobs in index order; DOPR54
+ * step estimates in a kernel of their own before the integrator).  Device
+ * scratch of the session: a few words per work item.  options->schedule is
+ * ignored.  Returns after the work is complete (it synchronises `stream`). */
 int drift_ragged_device(drift_session* session, const drift_job* jobs, size_t njobs,
                         const drift_potential* potentials, size_t npotentials,
                         const drift_launch_options* options, void* stream);
