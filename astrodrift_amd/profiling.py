@@ -149,15 +149,17 @@ def run_dopr54_uniform(n: int) -> int:
     return 0
 
 
-def derived(c: dict[str, float]) -> list[tuple[str, str, str]]:
-    """Ratios of an SQ counter set (whichever inputs are present)."""
+def derived(c: dict[str, float], waves: int = 2) -> list[tuple[str, str, str]]:
+    """Ratios of an SQ counter set (whichever inputs are present); `waves` is
+    the kernel's resident waves per SIMD."""
     rows = []
     if "SQ_INSTS_SALU" in c and "SQ_INSTS_VALU" in c:
         rows.append(("SALU per VALU instruction", f"{c['SQ_INSTS_SALU'] / c['SQ_INSTS_VALU']:.3f}",
                      "INSTS_SALU / INSTS_VALU"))
     if "SQ_ACTIVE_INST_VALU" in c and "SQ_WAVE_CYCLES" in c:
-        rows.append(("VALU busy", f"{100 * c['SQ_ACTIVE_INST_VALU'] / (c['SQ_WAVE_CYCLES'] / 2):.1f}%",
-                     "ACTIVE_INST_VALU / (WAVE_CYCLES / 2 waves per SIMD)"))
+        rows.append(("VALU busy",
+                     f"{100 * c['SQ_ACTIVE_INST_VALU'] / (c['SQ_WAVE_CYCLES'] / waves):.1f}%",
+                     f"ACTIVE_INST_VALU / (WAVE_CYCLES / {waves} waves per SIMD)"))
     if "SQ_THREAD_CYCLES_VALU" in c and "SQ_ACTIVE_INST_VALU" in c:
         rows.append(("active lanes per VALU instruction",
                      f"{100 * c['SQ_THREAD_CYCLES_VALU'] / (64 * c['SQ_ACTIVE_INST_VALU']):.1f}%",
@@ -237,7 +239,7 @@ def long_dispatches(trace: Path, min_ms: float) -> None:
         print(f"| {k} | `{short}` | {ms:.3f} |")
 
 
-def kernel_counters(dirs: list[str], needle: str) -> int:
+def kernel_counters(dirs: list[str], needle: str, waves: int = 2) -> int:
     """SQ counters of every dispatch whose name contains `needle`, per counter
     folder, merged per dispatch order, with derived ratios."""
     merged: dict[str, dict[str, float]] = {}
@@ -257,7 +259,7 @@ def kernel_counters(dirs: list[str], needle: str) -> int:
         print()
         print("| quantity | value | how |")
         print("|---|---|---|")
-        for q, v, how in derived(c):
+        for q, v, how in derived(c, waves):
             print(f"| {q} | {v} | {how} |")
         print()
     return 0
@@ -334,12 +336,14 @@ def main() -> int:
     p.add_argument("--kernel", metavar="NAME",
                    help="with --summarize: the SQ counters of the dispatches whose name contains "
                         "NAME, from every folder given, with derived ratios")
+    p.add_argument("--waves", type=int, default=2,
+                   help="with --kernel: the kernel's resident waves per SIMD (for VALU busy)")
     p.add_argument("--min-ms", type=float, default=0.0,
                    help="with --summarize and a trace folder alone: only dispatches of at "
                         "least this many ms, grouped and in launch order")
     args = p.parse_args()
     if args.summarize and args.kernel:
-        return kernel_counters(args.summarize, args.kernel)
+        return kernel_counters(args.summarize, args.kernel, args.waves)
     if args.summarize:
         if len(args.summarize) == 2:
             p.error("--summarize takes the trace folder, then the fetch and write folders")
