@@ -211,9 +211,10 @@ dop853_plain_kernel(PlainArgs a) {
 // Waves per SIMD of the DOPR54 kernels.  Without a table the step loop needs
 // under 104 registers (the step estimate runs out of line), so four waves fit
 // and hide the latency of the short force evaluations (a point-mass launch
-// runs 14-24% faster than at three); with the Milky-Way table (30 KB in FP64,
-// 6 KB in float) the kernels need 154-160 VGPRs and run three; the moving
-// origins' registers allow two.
+// runs 14-24% faster than at three).  With the Milky-Way table the kernels
+// need 154-160 VGPRs: the float table's kernel runs three waves, the FP64
+// one two (at three it ran 262,144 orbits 10% slower).  The moving origins'
+// registers allow two.
 template <int KIND, class FT>
 constexpr int dopr54_waves() {
     return (KIND & POT_MOVING) ? 2 : kind_has_table(KIND) ? (sizeof(FT) == 8 ? 2 : 3) : 4;
