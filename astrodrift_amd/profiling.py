@@ -66,7 +66,9 @@ def run(args) -> int:
         return run_ragged(args.ragged_jobs)
     if args.dopr54_uniform > 0:
         return run_dopr54_uniform(args.dopr54_uniform)
-    w = workloads.mw_disk(n=args.n)
+    rtol = args.rtol if args.rtol else None
+    w = workloads.mw_disk(n=args.n) if rtol is None else workloads.mw_disk(n=args.n, rtol=rtol,
+                                                                          atol=rtol)
     s0 = torch.from_numpy(w.state0).to(dev)
     tt = torch.from_numpy(w.times).to(dev)
     out = torch.empty((w.nt, w.n, 6), dtype=torch.float64, device=dev)
@@ -74,7 +76,9 @@ def run(args) -> int:
     lt = math.log(w.rtol)
     _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(), out.data_ptr(),
                           w.n, w.nt, lt, lt, w.potential, st.data_ptr(), schedule=args.schedule,
-                          block_size=args.block)
+                          block_size=args.block,
+                          precision=_native.PRECISION_MIXED if args.mixed else _native.PRECISION_FP64,
+                          staging=_native.STAGING_STAGED if args.staged else _native.STAGING_AUTO)
     torch.cuda.synchronize()
     acc = int(st[:, 0].sum().item())
     print(f"[profiling] integrator: n={w.n} nt={w.nt} accepted={acc} "
@@ -323,6 +327,11 @@ def main() -> int:
     p.add_argument("--schedule", type=int, default=1, help="0 plain, 1 persistent")
     p.add_argument("--block", type=int, default=0,
                    help="threads per block (0: the library default, 512 for this launch)")
+    p.add_argument("--rtol", type=float, default=0.0,
+                   help="rtol = atol of the 1M-orbit launch (0: its 1e-12)")
+    p.add_argument("--mixed", action="store_true", help="mixed precision (FP32 stage forces)")
+    p.add_argument("--staged", action="store_true",
+                   help="staged output (orbit-blocked samples, then the transpose)")
     p.add_argument("--copy-gib", type=float, default=2.0, help="0 skips the calibration copy")
     p.add_argument("--summarize", nargs="+", metavar="DIR",
                    help="trace folder, then optionally the fetch and write folders and more "
