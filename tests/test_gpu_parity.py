@@ -218,6 +218,36 @@ def test_edge_shapes(gpu, method):
         assert ep < REL_TOL and ev < REL_TOL, (times.size, ep, ev)
 
 
+@pytest.mark.parametrize("nt", [1024, 1025])
+def test_time_grid_in_lds_or_global_memory(gpu, nt):
+    """DOP853 kernels copy a launch's time grid into LDS when it has at most
+    1,024 outputs and read it from global memory beyond: on both sides of the
+    bound the plain and the persistent schedule match the oracle on the same
+    Milky-Way orbits, and a host-buffer launch runs the grid launch_shape
+    reports."""
+    torch = pytest.importorskip("torch")
+    w = workloads.mw_disk(n=96, t_end=10.0, steps=nt, rtol=1e-10, atol=1e-10)
+    tol = math.log(w.rtol)
+    cpu, _ = oracle_lib.batch(_native.METHOD_DOP853, w.state0, w.times, tol, tol, w.potential,
+                              nthreads=8)
+    dev = torch.device("cuda:0")
+    s0 = torch.from_numpy(w.state0).to(dev)
+    tt = torch.from_numpy(w.times).to(dev)
+    for schedule in (_native.SCHEDULE_PLAIN, _native.SCHEDULE_PERSISTENT):
+        out = torch.full((w.nt, w.n, 6), float("nan"), dtype=torch.float64, device=dev)
+        _native.launch_device(_native.METHOD_DOP853, s0.data_ptr(), tt.data_ptr(), out.data_ptr(),
+                              w.n, w.nt, tol, tol, w.potential, schedule=schedule)
+        torch.cuda.synchronize()
+        ep, ev = rel_err(out.cpu().numpy(), cpu)
+        assert ep < REL_TOL and ev < REL_TOL, (nt, schedule, ep, ev)
+    _, _, code = _native.launch_host(_native.METHOD_DOP853, w.state0, w.times, tol, tol,
+                                     w.potential)
+    assert code == 0
+    rep = _native.last_launch()
+    assert (rep["grid"], rep["block"]) == _native.launch_shape(_native.METHOD_DOP853, w.n,
+                                                               w.potential)
+
+
 def test_step_limit_is_flagged(gpu):
     """A host-forced tiny attempt limit ends the orbit and flags it."""
     torch = pytest.importorskip("torch")
