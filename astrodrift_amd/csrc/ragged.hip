@@ -876,23 +876,32 @@ extern "C" int drift_tune_ragged(drift_session* session, const drift_job* jobs, 
             // descriptor upload and the DOPR54 step estimates cost the same at
             // every block size, and on a short prefix they would drown the
             // differences
+            //
+            // Each block size runs twice and keeps its faster time.  A smaller
+            // block than 256 must beat 256 by 3%: on the prefix, which lacks
+            // the orbits' long middles, 64 once came out ahead by less and the
+            // full mix's DOPR54 jobs then ran 25% slower than at 256.
             RaggedScratch& sc = session->ragged[method];
             sc.time_kernel = true;
-            for (int block : {64, 128, 256}) {
+            for (int block : {256, 128, 64}) {
                 o.block_size = block;
-                // returns once the launch has finished (it synchronises the stream)
-                rc = drift_ragged_device(session, sub.data(), sub.size(), potentials, npotentials,
-                                         &o, stream);
-                float ms = 0.0f;
-                if (rc == DRIFT_OK &&
-                    hipEventElapsedTime(&ms, sc.kernel_start, sc.kernel_end) != hipSuccess)
-                    rc = fail(DRIFT_ERR_DEVICE, "hipEventElapsedTime failed");
-                if (rc != DRIFT_OK) {
-                    sc.time_kernel = false;
-                    return rc;
+                double t = INFINITY;
+                for (int rep = 0; rep < 2; ++rep) {
+                    // returns once the launch has finished (it synchronises the stream)
+                    rc = drift_ragged_device(session, sub.data(), sub.size(), potentials,
+                                             npotentials, &o, stream);
+                    float ms = 0.0f;
+                    if (rc == DRIFT_OK &&
+                        hipEventElapsedTime(&ms, sc.kernel_start, sc.kernel_end) != hipSuccess)
+                        rc = fail(DRIFT_ERR_DEVICE, "hipEventElapsedTime failed");
+                    if (rc != DRIFT_OK) {
+                        sc.time_kernel = false;
+                        return rc;
+                    }
+                    t = std::min(t, (double)ms);
                 }
-                if (ms < best_s) {
-                    best_s = ms;
+                if (block == 256 ? t < best_s : t < 0.97 * best_s) {
+                    best_s = t;
                     best = block;
                 }
             }

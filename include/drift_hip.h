@@ -332,8 +332,9 @@ int drift_ragged_device(drift_session* session, const drift_job* jobs, size_t nj
 
 /* Autotuner for drift_ragged_device: integrates every job over its first
  * prefix_samples outputs (0 = 5% of its grid, at least 2) with each block size
- * in {64, 128, 256}, one method at a time, and stores the fastest per
- * (precision, method) in the session; ragged launches of the session whose
+ * in {64, 128, 256}, one method at a time, twice each (the integrator kernel's
+ * faster time counts), and stores the fastest per (precision, method) in the
+ * session, 128 or 64 only if 3% faster than 256; ragged launches of the session whose
  * options->block_size is 0 use it (256 while untuned).  best_blocks (nullable)
  * receives [DOPR54, DOP853] (0 = no job of that method).  Needs the prefixes'
  * outputs as session scratch and synchronises `stream`.  No counterpart in the
