@@ -564,7 +564,8 @@ constexpr int table_words() {
     return kind_has_table(KIND) ? (sizeof(FT) == 8 ? TABLE_DOUBLES : TABLE32_FLOATS) : 1;
 }
 
-// Stash slots per wavefront: what the LDS holds beside the table, at most 56.
+// Stash slots per wavefront: what the LDS holds beside the table (and the
+// time grid's copy, LT), at most 56.
 template <int KIND, class FT, bool LT = false>
 constexpr int packed_slots() {
     constexpr int free_bytes = kLdsBytes - table_words<KIND, FT>() * (int)sizeof(FT) -
